@@ -1,0 +1,151 @@
+/*
+ * libsnapgpu — C ABI of the MI355X (gfx950) kernels behind snapatac2_amd.
+ *
+ * Conventions
+ *   - Every pointer named d_* is device memory; h_* is host memory.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *   - Every entry point returns 0 (SNAPGPU_OK) or a SNAPGPU_ERR_* code; the
+ *     message of the last failure on the calling thread is returned by
+ *     snapgpu_last_error().
+ *   - The library allocates no device memory: callers pass outputs and, where
+ *     an entry point needs scratch space, a workspace whose size is given by the
+ *     matching *_workspace_bytes() query.
+ *   - Kernels are deterministic: the same inputs give bitwise-equal outputs.
+ */
+#ifndef SNAPGPU_H
+#define SNAPGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum snapgpu_status {
+  SNAPGPU_OK = 0,
+  SNAPGPU_ERR_ARG = 1,      /* bad argument (shape, enum value, null pointer) */
+  SNAPGPU_ERR_HIP = 2,      /* HIP runtime failure                            */
+  SNAPGPU_ERR_DATA = 3,     /* input breaks an invariant the reference panics on */
+  SNAPGPU_ERR_CAPACITY = 4  /* input exceeds a documented kernel limit        */
+};
+
+/* Counting strategies of the tile matrix.
+ * Reference: snapatac2-core/src/feature_count/counter.rs:19-36 (CountingStrategy)
+ * and data_iter.rs:312-415 (gen_mat_single / gen_mat_pair). */
+enum snapgpu_strategy {
+  SNAPGPU_INSERTION = 0, /* "insertion": +1 at both insertion bins            */
+  SNAPGPU_FRAGMENT = 1,  /* "fragment": +1 at every bin the fragment covers    */
+  SNAPGPU_PIC = 2,       /* "paired-insertion": second +1 only if a new bin    */
+  SNAPGPU_SINGLE = 3     /* single-end reads (obsm['fragment_single'])         */
+};
+
+const char* snapgpu_last_error(void);
+int snapgpu_abi_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Prefix sums                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* Bytes of workspace snapgpu_scan_inclusive_i64 needs for n elements. */
+int64_t snapgpu_scan_workspace_bytes(int64_t n);
+
+/* In-place inclusive prefix sum of d_data[0..n). Used to turn per-row counts
+ * into CSR row pointers (the reference builds them serially in anndata-rs
+ * to_csr_data, called at data_iter.rs:413). */
+int snapgpu_scan_inclusive_i64(int64_t* d_data, int64_t n, void* d_workspace,
+                               int64_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Path A: fragment -> cell x bin CSR ("tile matrix")                        */
+/* Replaces mk_tile_matrix (snapatac2-python/src/preprocessing.rs:286-345)   */
+/* -> create_tile_matrix (feature_count/matrix.rs:26-93) -> gen_mat_pair /   */
+/* gen_mat_single (feature_count/data_iter.rs:312-415).                      */
+/* ------------------------------------------------------------------------ */
+
+/* Size in bytes of the packed chromosome table for n_chrom chromosomes. */
+int64_t snapgpu_tile_table_bytes(int32_t n_chrom);
+
+/* Host-only. Packs the chromosome table the tile kernels read.
+ *   h_chrom_len[n_chrom]: lengths in the stored order of
+ *     uns['reference_sequences'] (GenomeBaseIndex::new, genome.rs:296-312);
+ *   h_excluded[n_chrom]: 1 = chromosome excluded from the output, which
+ *     renumbers the bins of the others (FragmentData::get_gindex,
+ *     data_iter.rs:156-173);
+ * Writes the table to h_out (snapgpu_tile_table_bytes bytes; copy it to the
+ * device before use) and the number of output bins to *n_bins
+ * (GenomeBaseIndex::with_step + len, genome.rs:341-384). */
+int snapgpu_tile_table_pack(const uint64_t* h_chrom_len, const uint8_t* h_excluded,
+                            int32_t n_chrom, uint32_t bin_size, void* h_out,
+                            uint64_t* n_bins);
+
+/* Pass 1: number of stored (bin, count) entries of each cell.
+ *   d_cell_ptr[n_cells+1]: fragment row pointers of obsm['fragment_paired'|
+ *     'fragment_single'] (import.rs:117-159);
+ *   d_pos[F]: 0-based genome base index of each fragment (import.rs:161-210);
+ *   d_len[F]: fragment length (paired) or NULL (single-end, whose value is not
+ *     used by gen_mat_single);
+ *   min_len/max_len: inclusive size filter (0 / UINT32_MAX = none);
+ *   d_row_nnz[n_cells]: output;
+ *   d_status[8]: int32 device scratch for error reporting;
+ *   *level: in = first capacity level to try (0), out = level that succeeded
+ *     (pass it to snapgpu_tile_fill).
+ * Synchronises `stream` once per level tried. */
+int snapgpu_tile_count(const uint64_t* d_cell_ptr, const uint32_t* d_pos,
+                       const uint32_t* d_len, int64_t n_cells, const void* d_table,
+                       int32_t n_chrom, uint32_t bin_size, int32_t strategy,
+                       uint32_t min_len, uint32_t max_len, int64_t* d_row_nnz,
+                       int32_t* d_status, int32_t* level, void* stream);
+
+/* Pass 2: writes the CSR columns (ascending per row) and counts.
+ *   d_indptr[n_cells+1]: row pointers (scan of pass-1 counts);
+ *   d_out_idx: int32 columns if index_bits == 32, int64 if 64;
+ *   d_out_val: uint32 counts. */
+int snapgpu_tile_fill(const uint64_t* d_cell_ptr, const uint32_t* d_pos,
+                      const uint32_t* d_len, int64_t n_cells, const void* d_table,
+                      int32_t n_chrom, uint32_t bin_size, int32_t strategy,
+                      uint32_t min_len, uint32_t max_len, int32_t level,
+                      const int64_t* d_indptr, void* d_out_idx, int32_t index_bits,
+                      uint32_t* d_out_val, int32_t* d_status, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Feature selection input                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* d_out[j] += sum of the stored counts of column j (uint64; zero it first).
+ * Integer sums, so the order of accumulation does not matter. Replaces the
+ * chunked column sums of select_features
+ * (snapatac2-python/python/snapatac2/preprocessing/_basic.py:1053-1055). */
+int snapgpu_csr_col_sum_u32(const void* d_indices, int32_t index_bits, const uint32_t* d_values,
+                            int64_t nnz, uint64_t* d_out, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic workload generator (no reference counterpart: it produces the   */
+/* obsm['fragment_paired'] encoding of import.rs:161-210 from a seed).       */
+/* ------------------------------------------------------------------------ */
+
+/* Fragments per cell for cells first_cell .. first_cell+n_cells-1:
+ * round(exp(mu + sigma*N(0,1))) clipped to [min_frags, max_frags]. */
+int snapgpu_synth_counts(uint64_t seed, int64_t first_cell, int64_t n_cells, double mu,
+                         double sigma, int64_t min_frags, int64_t max_frags, int64_t* d_counts,
+                         void* stream);
+
+int64_t snapgpu_synth_workspace_bytes(int64_t n_frags);
+
+/* Fragments of a chunk of cells (< 2^20 cells, < 2^31 fragments), sorted per
+ * cell by (pos, len). d_cell_ptr: local row pointers of the chunk (from
+ * snapgpu_synth_counts + scan); d_chrom_start[n_chrom+1]: cumulative
+ * chromosome starts; d_type_cdf[n_types*n_peaks]: per-type cumulative peak
+ * activity. */
+int snapgpu_synth_fragments(uint64_t seed, int64_t first_cell, int64_t n_cells,
+                            const int64_t* d_cell_ptr, int64_t n_frags,
+                            const uint64_t* d_chrom_start, int32_t n_chrom,
+                            const uint32_t* d_peak_centre, int32_t n_peaks,
+                            const float* d_type_cdf, int32_t n_types, double p_uniform,
+                            double peak_sd, uint32_t* d_pos, uint32_t* d_len,
+                            void* d_workspace, int64_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SNAPGPU_H */

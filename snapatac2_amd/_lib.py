@@ -1,0 +1,92 @@
+"""ctypes binding of libsnapgpu.so (the C ABI declared in include/snapgpu.h).
+
+The library is the only compute path: if it is missing, or no ROCm GPU is
+present, every operation raises instead of falling back to CPU code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libsnapgpu.so"
+
+OK, ERR_ARG, ERR_HIP, ERR_DATA, ERR_CAPACITY = 0, 1, 2, 3, 4
+
+_p = C.c_void_p
+_i32, _i64, _u32, _u64 = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64
+
+# name -> (restype, argtypes)
+SIGNATURES: dict[str, tuple] = {
+    "snapgpu_last_error": (C.c_char_p, []),
+    "snapgpu_abi_version": (C.c_int, []),
+    "snapgpu_scan_workspace_bytes": (_i64, [_i64]),
+    "snapgpu_scan_inclusive_i64": (C.c_int, [_p, _i64, _p, _i64, _p]),
+    "snapgpu_tile_table_bytes": (_i64, [_i32]),
+    "snapgpu_tile_table_pack": (C.c_int, [_p, _p, _i32, _u32, _p, C.POINTER(_u64)]),
+    "snapgpu_tile_count": (C.c_int, [_p, _p, _p, _i64, _p, _i32, _u32, _i32, _u32, _u32, _p, _p,
+                                     C.POINTER(_i32), _p]),
+    "snapgpu_tile_fill": (C.c_int, [_p, _p, _p, _i64, _p, _i32, _u32, _i32, _u32, _u32, _i32, _p,
+                                    _p, _i32, _p, _p, _p]),
+    "snapgpu_csr_col_sum_u32": (C.c_int, [_p, _i32, _p, _i64, _p, _p]),
+    "snapgpu_synth_counts": (C.c_int, [_u64, _i64, _i64, C.c_double, C.c_double, _i64, _i64, _p,
+                                       _p]),
+    "snapgpu_synth_workspace_bytes": (_i64, [_i64]),
+    "snapgpu_synth_fragments": (C.c_int, [_u64, _i64, _i64, _p, _i64, _p, _i32, _p, _i32, _p,
+                                          _i32, C.c_double, C.c_double, _p, _p, _p, _i64, _p]),
+}
+
+
+class SnapGpuError(RuntimeError):
+    """A failure reported by libsnapgpu."""
+
+
+class PanicException(SnapGpuError):
+    """Raised where the reference Rust code panics on invalid data."""
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m snapatac2_amd.build` "
+                "(there is no CPU fallback)")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc == OK:
+        return
+    msg = lib().snapgpu_last_error().decode(errors="replace")
+    if rc == ERR_ARG:
+        raise ValueError(msg)
+    if rc == ERR_DATA:
+        raise PanicException(msg)
+    raise SnapGpuError(msg)
+
+
+def require_gpu():
+    """Returns torch and the current device; raises when no ROCm GPU is visible."""
+    import torch
+    if not torch.cuda.is_available():
+        raise SnapGpuError("snapatac2_amd needs a ROCm GPU (MI355X / gfx950); none is visible")
+    return torch
+
+
+def stream_ptr(torch_mod=None) -> int:
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+def ptr(t) -> int:
+    """Device address of a torch tensor (0 for None)."""
+    return 0 if t is None else t.data_ptr()

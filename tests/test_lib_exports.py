@@ -1,0 +1,47 @@
+"""The C-ABI library loads (no GPU needed) and exports every entry point that
+include/snapgpu.h declares; the ctypes table covers them all."""
+import ctypes
+import os
+import re
+
+from snapatac2_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            src = open(os.path.join(ROOT, "include", fn)).read()
+            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+            names |= set(re.findall(r"\b(snapgpu_\w+)\s*\(", src))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    syms = declared_symbols()
+    assert len(syms) >= 8
+    missing = [s for s in sorted(syms) if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_ctypes_table_matches_header():
+    assert declared_symbols() == set(_lib.SIGNATURES)
+
+
+def test_host_only_entry_points():
+    L = _lib.lib()
+    assert L.snapgpu_abi_version() >= 1
+    import numpy as np
+    lens = np.asarray([1000, 600, 100], dtype=np.uint64)
+    excl = np.asarray([0, 0, 1], dtype=np.uint8)
+    buf = np.zeros(L.snapgpu_tile_table_bytes(3), dtype=np.uint8)
+    nb = ctypes.c_uint64()
+    assert L.snapgpu_tile_table_pack(lens.ctypes.data, excl.ctypes.data, 3, 100,
+                                     buf.ctypes.data, ctypes.byref(nb)) == 0
+    assert nb.value == 16
+    assert L.snapgpu_tile_table_pack(lens.ctypes.data, excl.ctypes.data, 3, 0,
+                                     buf.ctypes.data, ctypes.byref(nb)) == _lib.ERR_ARG
+    assert b"bin_size" in L.snapgpu_last_error()
