@@ -144,6 +144,27 @@ int snapgpu_synth_fragments(uint64_t seed, int64_t first_cell, int64_t n_cells,
                             double peak_sd, uint32_t* d_pos, uint32_t* d_len,
                             void* d_workspace, int64_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Path C: exact k-nearest-neighbour graph                                   */
+/* Replaces nearest_neighbour_graph (snapatac2-python/src/knn.rs:7-16 ->      */
+/* snapatac2-core/src/utils/knn.rs:9-32, 58-78).                             */
+/* ------------------------------------------------------------------------ */
+
+int64_t snapgpu_knn_workspace_bytes(int64_t n, int32_t d, int32_t k);
+
+/* d_X: n x d row-major FP64 (leading dimension ld), d <= 30, 1 <= k <= min(100, n-1).
+ * mode 0: FP32 SIMT candidate distances; mode 1: BF16 MFMA (hi/lo split, FP32
+ * accumulation). Both rescore candidates in exact FP64 and prove the result
+ * with a certificate (failing rows are recomputed exactly), so the neighbour
+ * sets equal FP64 brute force with ties broken by index.
+ * Outputs: d_nbr[n*k] (int32) and d_dist[n*k] (FP64 Euclidean distance), each
+ * row sorted by neighbour index (knn.rs:67). flags bit 0: send every row
+ * through the exact fallback (testing). *n_fallback: rows that needed it.
+ * Synchronises `stream` once. */
+int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, int32_t mode,
+                int32_t* d_nbr, double* d_dist, void* d_workspace, int64_t workspace_bytes,
+                int32_t flags, int64_t* n_fallback, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,6 +33,9 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_synth_workspace_bytes": (_i64, [_i64]),
     "snapgpu_synth_fragments": (C.c_int, [_u64, _i64, _i64, _p, _i64, _p, _i32, _p, _i32, _p,
                                           _i32, C.c_double, C.c_double, _p, _p, _p, _i64, _p]),
+    "snapgpu_knn_workspace_bytes": (_i64, [_i64, _i32, _i32]),
+    "snapgpu_knn": (C.c_int, [_p, _i64, _i32, _i64, _i32, _i32, _p, _p, _p, _i64, _i32,
+                              C.POINTER(_i64), _p]),
 }
 
 

@@ -1,5 +1,92 @@
-"""Drop-in for snapatac2.pp.knn (filled in below)."""
+"""Drop-in for snapatac2.pp.knn (exact kd-tree method).
+
+Reference: snapatac2-python/python/snapatac2/preprocessing/_knn.py:10-87 and
+nearest_neighbour_graph (snapatac2-python/src/knn.rs:7-16,
+snapatac2-core/src/utils/knn.rs:9-32,58-78).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Literal
+
+import numpy as np
+from scipy.sparse import csr_matrix
+
+from .. import _lib
+
+_MODES = {"fp32": 0, "bf16": 1}
 
 
-def knn(*args, **kwargs):
-    raise NotImplementedError
+def nearest_neighbour_graph(data, k: int, precision: str = "bf16", force_fallback: bool = False,
+                            return_stats: bool = False):
+    """Exact kNN graph of the rows of `data` (n x d, d <= 30): scipy CSR n x n,
+    float64 Euclidean distances, k entries per row sorted by neighbour index,
+    self excluded by index. `data` may be a numpy array or a CUDA torch tensor."""
+    torch = _lib.require_gpu()
+    L = _lib.lib()
+    if precision not in _MODES:
+        raise ValueError("precision must be 'fp32' or 'bf16'")
+    if isinstance(data, torch.Tensor):
+        X = data.to(device="cuda", dtype=torch.float64).contiguous()
+    else:
+        X = torch.from_numpy(np.ascontiguousarray(data, dtype=np.float64)).cuda()
+    if X.dim() != 2:
+        raise ValueError("data must be 2-dimensional")
+    n, d = X.shape
+    kk = min(int(k), n - 1)
+    if kk < 1:
+        # the kd-tree returns no neighbours for a single point
+        out = csr_matrix((n, n), dtype=np.float64)
+        return (out, {"fallback_rows": 0}) if return_stats else out
+    nbr = torch.empty((n, kk), dtype=torch.int32, device=X.device)
+    dist = torch.empty((n, kk), dtype=torch.float64, device=X.device)
+    ws_b = int(L.snapgpu_knn_workspace_bytes(n, d, kk))
+    ws = torch.empty(ws_b, dtype=torch.uint8, device=X.device)
+    nfb = C.c_int64(0)
+    _lib.check(L.snapgpu_knn(X.data_ptr(), n, d, d, kk, _MODES[precision], nbr.data_ptr(),
+                             dist.data_ptr(), ws.data_ptr(), ws_b, 1 if force_fallback else 0,
+                             C.byref(nfb), _lib.stream_ptr()))
+    indptr = np.arange(0, n * kk + 1, kk, dtype=np.int64)
+    out = csr_matrix((dist.cpu().numpy().ravel(), nbr.cpu().numpy().ravel(), indptr), shape=(n, n))
+    return (out, {"fallback_rows": int(nfb.value)}) if return_stats else out
+
+
+def knn(
+    adata,
+    n_neighbors: int = 50,
+    use_dims: int | list[int] | None = None,
+    use_rep: str = "X_spectral",
+    method: Literal["kdtree", "hora", "pynndescent"] = "kdtree",
+    inplace: bool = True,
+    random_state: int = 0,
+    precision: Literal["fp32", "bf16"] = "bf16",
+) -> csr_matrix | None:
+    """Compute a neighborhood graph of observations (drop-in for snapatac2.pp.knn).
+
+    method='kdtree' (exact) runs on the GPU; `precision` selects the candidate
+    distance arithmetic, and both settings return the exact FP64 neighbour sets.
+    The approximate methods ('hora', 'pynndescent') are not provided.
+    """
+    if hasattr(adata, "obsm"):
+        data = adata.obsm[use_rep]
+    else:
+        inplace = False
+        data = adata
+    size = data.numel() if hasattr(data, "numel") else np.size(data)
+    if size == 0:
+        raise ValueError("matrix is empty")
+    if use_dims is not None:
+        if isinstance(use_dims, int):
+            data = data[:, :use_dims]
+        else:
+            data = data[:, use_dims]
+    if method == "kdtree":
+        adj = nearest_neighbour_graph(data, n_neighbors, precision=precision)
+    elif method in ("hora", "pynndescent"):
+        raise NotImplementedError(f"approximate method {method!r} is not provided; use 'kdtree'")
+    else:
+        raise ValueError("method must be one of 'hora', 'pynndescent', 'kdtree'")
+    if inplace:
+        adata.obsp["distances"] = adj
+        return None
+    return adj

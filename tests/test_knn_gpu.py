@@ -1,0 +1,79 @@
+"""Path C on the GPU: neighbour sets and distances vs the FP64 brute-force oracle."""
+import numpy as np
+import pytest
+
+from oracle import knn as oknn
+
+pytestmark = pytest.mark.gpu
+
+
+def clustered(n, d=30, n_clusters=10, seed=0):
+    rng = np.random.default_rng(seed)
+    centres = 3.0 * rng.standard_normal((n_clusters, d))
+    lab = rng.integers(0, n_clusters, n)
+    X = centres[lab] + rng.standard_normal((n, d))
+    return X / np.linalg.norm(X, axis=1, keepdims=True)
+
+
+def _check(g, X, k):
+    o = oknn.knn_graph(X, k)
+    assert g.shape == o.shape
+    assert np.array_equal(g.indptr, o.indptr)
+    assert np.array_equal(g.indices, o.indices)
+    # distances: exact FP64 rescoring with the oracle's formula -> identical bits
+    assert np.array_equal(g.data, o.data)
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+@pytest.mark.parametrize("n,d,k", [(5000, 30, 50), (3000, 17, 25), (1000, 30, 100), (100, 5, 50)])
+def test_clustered_vs_oracle(torch_gpu, precision, n, d, k):
+    from snapatac2_amd.pp._knn import nearest_neighbour_graph
+    X = clustered(n, d, seed=n + d)
+    g, st = nearest_neighbour_graph(X, k, precision=precision, return_stats=True)
+    _check(g, X, k)
+    assert st["fallback_rows"] <= n // 100 + 1
+
+
+def test_spectral_scale_embedding(torch_gpu):
+    # embeddings like X_spectral: tiny values (eigenvectors * sqrt(eigenvalue))
+    from snapatac2_amd.pp._knn import nearest_neighbour_graph
+    X = clustered(4000, 30, seed=9) * 1e-3
+    X[:, 20:] *= 1e-2
+    _check(nearest_neighbour_graph(X, 50), X, 50)
+
+
+def test_ties_and_duplicates(torch_gpu):
+    # integer lattice points: many exactly equal distances, plus duplicate rows
+    from snapatac2_amd.pp._knn import nearest_neighbour_graph
+    rng = np.random.default_rng(1)
+    X = rng.integers(0, 4, (1500, 3)).astype(np.float64)
+    for precision in ("bf16", "fp32"):
+        _check(nearest_neighbour_graph(X, 30, precision=precision), X, 30)
+
+
+def test_forced_fallback_is_exact(torch_gpu):
+    from snapatac2_amd.pp._knn import nearest_neighbour_graph
+    X = clustered(700, 30, seed=3)
+    g, st = nearest_neighbour_graph(X, 50, force_fallback=True, return_stats=True)
+    assert st["fallback_rows"] == 700
+    _check(g, X, 50)
+
+
+def test_knn_dropin(torch_gpu):
+    import snapatac2_amd as snap
+    X = clustered(800, 30, seed=5)
+    ad = snap.AnnData(n_obs=800, n_vars=1)
+    ad.obsm["X_spectral"] = X
+    snap.pp.knn(ad, n_neighbors=25)
+    _check(ad.obsp["distances"], X, 25)
+    g2 = snap.pp.knn(X, n_neighbors=25, use_dims=10)
+    _check(g2, X[:, :10], 25)
+    # determinism (reference test_reproducibility, test_tools.py:95-122)
+    g3 = snap.pp.knn(ad, n_neighbors=25, inplace=False)
+    assert (g3 != ad.obsp["distances"]).nnz == 0
+    with pytest.raises(ValueError, match="empty"):
+        snap.pp.knn(np.zeros((0, 5)))
+    with pytest.raises(ValueError, match="method"):
+        snap.pp.knn(ad, method="bogus")
+    tiny = snap.pp.knn(X[:5], n_neighbors=50)
+    assert tiny.nnz == 20
