@@ -165,6 +165,74 @@ int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, 
                 int32_t* d_nbr, double* d_dist, void* d_workspace, int64_t workspace_bytes,
                 int32_t flags, int64_t* n_fallback, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Path B: the normalized-cosine operator inside spectral()                  */
+/* Replaces spectral_embedding / idf / normalize / spectral_mf               */
+/* (snapatac2-python/src/embedding.rs:24-59, 131-191, 269-326) and the scipy  */
+/* LinearOperator matvec it hands to eigsh (embedding.rs:155-171).           */
+/* ------------------------------------------------------------------------ */
+
+/* Column selection by feature mask (spectral_embedding's slice_axis,
+ * embedding.rs:36-41). d_col_map[old column] = new column or -1. */
+int snapgpu_csr_select_count(const int64_t* d_indptr, const void* d_idx, int32_t index_bits,
+                             int64_t n, const int32_t* d_col_map, int64_t* d_row_nnz,
+                             void* stream);
+int snapgpu_csr_select_fill(const int64_t* d_indptr, const void* d_idx, int32_t index_bits,
+                            const uint32_t* d_val, int64_t n, const int32_t* d_col_map,
+                            const int64_t* d_new_indptr, int32_t* d_new_idx,
+                            uint32_t* d_new_val, void* stream);
+
+/* Stable CSR -> CSC transpose (rows ascending within each column, so column
+ * reductions add in the reference's CSR order, embedding.rs:139-144).
+ * h_indptr: host copy of the row pointers; entries are sorted chunk_nnz at a
+ * time. Workspace: snapgpu_csr_transpose_workspace_bytes(m, chunk_nnz). */
+int64_t snapgpu_csr_transpose_workspace_bytes(int64_t m, int64_t chunk_nnz);
+int snapgpu_csr_transpose(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_val,
+                          int64_t n, int64_t m, int64_t nnz, const int64_t* h_indptr,
+                          int64_t chunk_nnz, int64_t* d_colptr, int32_t* d_rowidx,
+                          uint32_t* d_cval, void* d_ws, int64_t ws_bytes, void* stream);
+
+/* IDF weights from the CSC column pointers (embedding.rs:269-286). d_ws16: 16
+ * bytes of device scratch. Synchronises `stream`. */
+int snapgpu_spectral_idf(const int64_t* d_colptr, int64_t m, int64_t n, double* d_w,
+                         void* d_ws16, void* stream);
+
+/* Row scales s = 1/||x o w|| (normalize, embedding.rs:314-326), column sums c
+ * and degrees d = X~ c - 1 (spectral_mf, embedding.rs:133-153); writes
+ * dinv = 1/d, r = s*sqrt(dinv) (FP64 and FP32) and w2 = w^2 (FP32).
+ * value_type: 0 = uint32 counts, 1 = float32 values (bit patterns in d_cnt).
+ * Fails with SNAPGPU_ERR_DATA if a non-empty row has d <= 0. Synchronises. */
+int snapgpu_spectral_prepare(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_cnt,
+                             int64_t n, const int64_t* d_colptr, const int32_t* d_rowidx,
+                             const uint32_t* d_ccnt, int64_t m, const double* d_w, double* d_s,
+                             double* d_c, double* d_dinv, double* d_r64, float* d_r32,
+                             float* d_w2, int32_t value_type, int32_t* d_status, void* stream);
+
+/* Y = A V for a block of b <= 32 columns (FP64 in/out, FP32 sparse products):
+ * A v = r o (X (w^2 o (X^T (r o v)))) - dinv o v, the eigen() matvec
+ * X @ (v.T @ X).T - D * v of embedding.rs:160-162. */
+int64_t snapgpu_spectral_apply_workspace_bytes(int64_t n, int64_t m);
+int snapgpu_spectral_apply(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_cnt,
+                           int64_t n, const int64_t* d_colptr, const int32_t* d_rowidx,
+                           const uint32_t* d_ccnt, int64_t m, const float* d_r32,
+                           const float* d_w2, const double* d_dinv, const double* d_V,
+                           int64_t ldv, int32_t b, double* d_Y, int64_t ldy, int32_t value_type,
+                           void* d_ws, int64_t ws_bytes, void* stream);
+
+/* FP64 tall-skinny products for the block Lanczos basis (the work ARPACK's
+ * dsaupd does inside eigsh, embedding.rs:164). Row-major operands.
+ * C[ka x kb] = A[n x ka]^T B[n x kb], fixed-order reduction; kb <= 32. */
+int64_t snapgpu_gemm_tn_workspace_bytes(int64_t n, int32_t ka, int32_t kb);
+int snapgpu_gemm_tn_f64(const double* d_A, int64_t lda, const double* d_B, int64_t ldb,
+                        int64_t n, int32_t ka, int32_t kb, double* d_C, void* d_ws,
+                        int64_t ws_bytes, void* stream);
+/* C[n x kb] = beta*C + alpha*A[n x ka] B[ka x kb]; kb <= 32. */
+int snapgpu_gemm_nn_f64(const double* d_A, int64_t lda, const double* d_B, int64_t n, int32_t ka,
+                        int32_t kb, double alpha, double beta, double* d_C, int64_t ldc,
+                        void* stream);
+/* Seeded uniform(-1/2, 1/2) values (Lanczos start block). */
+int snapgpu_random_f64(uint64_t seed, int64_t count, double* d_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,110 @@
+"""ORACLE (test infrastructure only) for path B, the spectral embedding.
+
+Restates, in numpy/scipy FP64:
+  idf            snapatac2-python/src/embedding.rs:269-286
+  normalize      embedding.rs:314-326
+  spectral_mf    embedding.rs:131-153 (degrees, scaling) and the embedded
+                 eigen() of :155-171, including its exact scipy call:
+                 numpy.random.seed(seed); eigsh(A, k=k, v0=numpy.random.rand(n)),
+                 sorted by descending eigenvalue
+  spectral       snapatac2-python/python/snapatac2/tools/_embedding.py:223-293
+                 (n_comps clamp, weighted_by_sd)
+The third-party arithmetic (ARPACK via scipy.sparse.linalg.eigsh, scipy 1.15 in
+this image, within the reference's pin scipy>=1.4,<2.0) is called exactly as the
+reference calls it.
+
+Pinned by tests/test_spectral_oracle.py: hand-derived weights and similarity
+entries (w, not sqrt(w); self-similarity removed), the closed-form eigenpair
+lambda = 1 with eigenvector D^1/2 1, dense numpy.linalg.eigh of the assembled
+operator, the IDF special cases, and bitwise determinism
+(snapatac2-python/tests/test_tools.py:95-122).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+
+def idf(X: sp.csr_matrix) -> np.ndarray:
+    n = X.shape[0]
+    df = np.bincount(X.indices, minlength=X.shape[1]).astype(np.float64)
+    if np.all(df == df[0]):
+        return np.ones(X.shape[1])
+    df = np.where(df == 0, 1.0, df)
+    df = np.where(df == n, n - 1.0, df)
+    return np.log(n / df)
+
+
+def normalize(X: sp.csr_matrix, w: np.ndarray) -> sp.csr_matrix:
+    X = sp.csr_matrix(X, dtype=np.float64, copy=True)
+    X.data *= w[X.indices]
+    for i in range(X.shape[0]):
+        a, b = X.indptr[i], X.indptr[i + 1]
+        if b > a:
+            norm = np.sqrt(np.sum(X.data[a:b] * X.data[a:b]))
+            X.data[a:b] /= norm
+    return X
+
+
+def operator_parts(X, features=None, feature_weights=None):
+    """(X^, dinv, w, Xnorm): the scaled matrix and D of eigen() (embedding.rs:131-153)."""
+    X = sp.csr_matrix(X)
+    if features is not None:
+        X = X[:, features]
+    X = sp.csr_matrix(X, dtype=np.float64)
+    X.sort_indices()
+    w = idf(X) if feature_weights is None else np.asarray(feature_weights, dtype=np.float64)
+    Xn = normalize(X, w)
+    col_sum = np.asarray(Xn.sum(axis=0)).ravel()
+    d = Xn @ col_sum - 1.0
+    with np.errstate(divide="ignore"):
+        dinv = 1.0 / d
+    Xh = sp.diags(np.sqrt(np.where(np.diff(Xn.indptr) > 0, dinv, 0.0))) @ Xn
+    return sp.csr_matrix(Xh), dinv, w, Xn
+
+
+def dense_operator(X, features=None, feature_weights=None) -> np.ndarray:
+    Xh, dinv, _, _ = operator_parts(X, features, feature_weights)
+    A = (Xh @ Xh.T).toarray()
+    A[np.diag_indices_from(A)] -= dinv
+    return A
+
+
+def eigen(Xh: sp.csr_matrix, D: np.ndarray, k: int, seed: int):
+    """The embedded eigen() of embedding.rs:155-171, verbatim in behaviour."""
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    np.random.seed(seed)
+
+    def f(v):
+        return Xh @ (v.T @ Xh).T - D * v
+
+    n = Xh.shape[0]
+    A = LinearOperator((n, n), matvec=f, dtype=np.float64)
+    evals, evecs = eigsh(A, k=k, v0=np.random.rand(n))
+    ix = evals.argsort()[::-1]
+    return evals[ix], evecs[:, ix]
+
+
+def spectral_embedding(X, features, n_components, random_state=0, feature_weights=None):
+    Xh, dinv, _, _ = operator_parts(X, features, feature_weights)
+    return eigen(Xh, dinv, n_components, random_state)
+
+
+def spectral(X, n_comps=30, features=None, random_state=0, weighted_by_sd=True,
+             feature_weights=None):
+    n_obs, n_vars = X.shape
+    n_comps = min(n_vars - 1, n_obs - 1, n_comps)
+    evals, evecs = spectral_embedding(X, features, n_comps, random_state, feature_weights)
+    if weighted_by_sd:
+        idx = [i for i in range(evals.shape[0]) if evals[i] > 0]
+        evals = evals[idx]
+        evecs = evecs[:, idx] * np.sqrt(evals)
+    return evals, evecs
+
+
+def subspace_angle(U: np.ndarray, V: np.ndarray) -> float:
+    """Largest principal angle (radians) between span(U) and span(V)."""
+    qu, _ = np.linalg.qr(U)
+    qv, _ = np.linalg.qr(V)
+    s = np.linalg.svd(qu.T @ qv, compute_uv=False)
+    return float(np.arccos(np.clip(s.min(), -1.0, 1.0)))

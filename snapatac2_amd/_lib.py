@@ -36,6 +36,22 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_knn_workspace_bytes": (_i64, [_i64, _i32, _i32]),
     "snapgpu_knn": (C.c_int, [_p, _i64, _i32, _i64, _i32, _i32, _p, _p, _p, _i64, _i32,
                               C.POINTER(_i64), _p]),
+    "snapgpu_csr_select_count": (C.c_int, [_p, _p, _i32, _i64, _p, _p, _p]),
+    "snapgpu_csr_select_fill": (C.c_int, [_p, _p, _i32, _p, _i64, _p, _p, _p, _p, _p]),
+    "snapgpu_csr_transpose_workspace_bytes": (_i64, [_i64, _i64]),
+    "snapgpu_csr_transpose": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _p, _i64, _p, _p, _p, _p,
+                                        _i64, _p]),
+    "snapgpu_spectral_idf": (C.c_int, [_p, _i64, _i64, _p, _p, _p]),
+    "snapgpu_spectral_prepare": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _p,
+                                           _p, _p, _p, _i32, _p, _p]),
+    "snapgpu_spectral_apply_workspace_bytes": (_i64, [_i64, _i64]),
+    "snapgpu_spectral_apply": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _p,
+                                         _i64, _i32, _p, _i64, _i32, _p, _i64, _p]),
+    "snapgpu_gemm_tn_workspace_bytes": (_i64, [_i64, _i32, _i32]),
+    "snapgpu_gemm_tn_f64": (C.c_int, [_p, _i64, _p, _i64, _i64, _i32, _i32, _p, _p, _i64, _p]),
+    "snapgpu_gemm_nn_f64": (C.c_int, [_p, _i64, _p, _i64, _i32, _i32, C.c_double, C.c_double,
+                                      _p, _i64, _p]),
+    "snapgpu_random_f64": (C.c_int, [_u64, _i64, _p, _p]),
 }
 
 

@@ -1,0 +1,634 @@
+// Path B: the normalized-cosine operator of spectral() and its preparation.
+//
+// Reference (snapatac2-python/src/embedding.rs):
+//   spectral_embedding :24-59  column slice by the feature mask, u32 -> f64,
+//                              weights = feature_weights or idf(X), normalize
+//   idf                :269-286 df_j = stored entries of column j; all equal ->
+//                              w = 1; else df 0 -> 1, df n -> n-1, w = ln(n / df)
+//   normalize          :314-326 x_ij *= w_j, then each row / its L2 norm
+//   spectral_mf        :131-153 c = column sums of X~, d = X~ c - 1,
+//                              dinv = 1/d, X^ = diag(sqrt(dinv)) X~
+//                      :155-171 matvec f(v) = X^ (X^T v) - dinv * v
+//
+// Storage: the selected matrix stays as integer counts, in CSR (rows = cells)
+// and in a CSC copy built by a deterministic stable transpose. Values are never
+// materialised: with r_i = s_i sqrt(dinv_i) (s_i = 1 / ||x_i o w||),
+//     A v = r o (X (w^2 o (X^T (r o v)))) - dinv o v,
+// so each operator application streams the column/row indices and counts once
+// in each direction and gathers 32-wide FP32 rows of the dense operand.
+// FP32 arithmetic inside the two sparse products, FP64 at the interfaces, fixed
+// summation order everywhere (no floating-point atomics): results are
+// bitwise reproducible.
+#include <hipcub/hipcub.hpp>
+
+#include "common.h"
+
+namespace snapgpu {
+namespace spectral {
+
+constexpr int kB = 32;  // dense block width (columns of V, Z, Y)
+
+// Stored values are uint32 counts (tile matrices) or FP32 bit patterns.
+template <bool FV>
+__device__ __forceinline__ double val64(const uint32_t* v, int64_t e) {
+  return FV ? (double)__uint_as_float(v[e]) : (double)v[e];
+}
+template <bool FV>
+__device__ __forceinline__ float val32(const uint32_t* v, int64_t e) {
+  return FV ? __uint_as_float(v[e]) : (float)v[e];
+}
+
+// ---------------------------------------------------------------------------
+// Column selection of a CSR count matrix (feature mask -> new column ids)
+// ---------------------------------------------------------------------------
+template <typename IdxT>
+__global__ void __launch_bounds__(256) select_count_kernel(const int64_t* indptr, const IdxT* idx,
+                                                           int64_t n, const int32_t* col_map,
+                                                           int64_t* row_nnz) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (row >= n) return;
+  int64_t cnt = 0;
+  for (int64_t e = indptr[row] + lane; e < indptr[row + 1]; e += 64) cnt += col_map[idx[e]] >= 0;
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+  if (lane == 0) row_nnz[row] = cnt;
+}
+
+template <typename IdxT>
+__global__ void __launch_bounds__(256) select_fill_kernel(const int64_t* indptr, const IdxT* idx,
+                                                          const uint32_t* val, int64_t n,
+                                                          const int32_t* col_map,
+                                                          const int64_t* new_indptr,
+                                                          int32_t* new_idx, uint32_t* new_val) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (row >= n) return;
+  int64_t out = new_indptr[row];
+  const int64_t e1 = indptr[row + 1];
+  for (int64_t e0 = indptr[row]; e0 < e1; e0 += 64) {
+    const int64_t e = e0 + lane;
+    int32_t nc = -1;
+    if (e < e1) nc = col_map[idx[e]];
+    const uint64_t m = ballot64(nc >= 0);
+    if (nc >= 0) {
+      const int64_t o = out + popc64(m & lanemask_lt(lane));
+      new_idx[o] = nc;
+      new_val[o] = val[e];
+    }
+    out += popc64(m);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stable CSR -> CSC transpose (rows stay ascending inside each column)
+// ---------------------------------------------------------------------------
+__global__ void col_count_kernel(const int32_t* idx, int64_t nnz, unsigned long long* counts) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += stride)
+    atomicAdd(&counts[idx[e]], 1ull);
+}
+
+// row id of every entry of rows [r0, r1) (entries [indptr[r0], indptr[r1]))
+__global__ void __launch_bounds__(256) entry_rows_kernel(const int64_t* indptr, int64_t r0, int64_t r1,
+                                                         uint32_t* row_of) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = r0 + (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (row >= r1) return;
+  const int64_t base = indptr[r0];
+  for (int64_t e = indptr[row] + lane; e < indptr[row + 1]; e += 64) row_of[e - base] = (uint32_t)row;
+}
+
+__global__ void iota_kernel(uint32_t* v, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) v[i] = (uint32_t)i;
+}
+
+// Places the sorted chunk: position = cursor[col] + rank of the element within
+// its column segment of this chunk.
+__global__ void scatter_chunk_kernel(const uint32_t* skeys, const uint32_t* spos, int64_t cn,
+                                     const uint32_t* row_of, const uint32_t* cval,
+                                     const int64_t* cursor, int32_t* rowidx, uint32_t* out_val) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cn; t += stride) {
+    const uint32_t col = skeys[t];
+    int64_t lo = 0, hi = t;  // first position with key == col
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (skeys[mid] < col) lo = mid + 1; else hi = mid;
+    }
+    const int64_t o = cursor[col] + (t - lo);
+    const uint32_t p = spos[t];
+    rowidx[o] = (int32_t)row_of[p];
+    out_val[o] = cval[p];
+  }
+}
+
+__global__ void cursor_advance_kernel(const uint32_t* keys, int64_t cn, int64_t* cursor) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cn; t += stride)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&cursor[keys[t]]), 1ull);
+}
+
+// ---------------------------------------------------------------------------
+// IDF, row scales, degrees
+// ---------------------------------------------------------------------------
+__global__ void df_minmax_kernel(const int64_t* colptr, int64_t m, unsigned long long* mm) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  unsigned long long lo = ~0ull, hi = 0;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride) {
+    const unsigned long long df = (unsigned long long)(colptr[j + 1] - colptr[j]);
+    lo = df < lo ? df : lo;
+    hi = df > hi ? df : hi;
+  }
+  atomicMin(&mm[0], lo);
+  atomicMax(&mm[1], hi);
+}
+
+__global__ void idf_kernel(const int64_t* colptr, int64_t m, int64_t n, const unsigned long long* mm,
+                           double* w) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  if (mm[0] == mm[1]) {  // all document frequencies equal (embedding.rs:273-274)
+    w[j] = 1.0;
+    return;
+  }
+  double df = (double)(colptr[j + 1] - colptr[j]);
+  const double nn = (double)n;
+  if (df == 0.0) df = 1.0;
+  else if (df == nn) df = nn - 1.0;
+  w[j] = log(nn / df);
+}
+
+// s_i = 1 / ||(count_ij * w_j)_j||_2 (0 for an empty row); one wave per row.
+template <bool FV>
+__global__ void __launch_bounds__(256) row_scale_kernel(const int64_t* indptr, const int32_t* idx,
+                                                        const uint32_t* cnt, int64_t n,
+                                                        const double* w, double* s) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (row >= n) return;
+  double acc = 0.0;
+  for (int64_t e = indptr[row] + lane; e < indptr[row + 1]; e += 64) {
+    const double x = val64<FV>(cnt, e) * w[idx[e]];
+    acc = fma(x, x, acc);
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0) s[row] = acc > 0.0 ? 1.0 / sqrt(acc) : 0.0;
+}
+
+// c_j = sum_i x~_ij = w_j * sum_i count_ij s_i; one wave per column (CSC).
+template <bool FV>
+__global__ void __launch_bounds__(256) colsum_kernel(const int64_t* colptr, const int32_t* rowidx,
+                                                     const uint32_t* cnt, int64_t m, const double* w,
+                                                     const double* s, double* c) {
+  const int lane = threadIdx.x & 63;
+  const int64_t col = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (col >= m) return;
+  double acc = 0.0;
+  for (int64_t e = colptr[col] + lane; e < colptr[col + 1]; e += 64)
+    acc = fma(val64<FV>(cnt, e), s[rowidx[e]], acc);
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0) c[col] = acc * w[col];
+}
+
+// d_i = x~_i . c - 1; dinv = 1/d; r = s sqrt(dinv). Flags rows with d <= 0 that
+// have entries (a cell sharing no feature with any other cell).
+template <bool FV>
+__global__ void __launch_bounds__(256) degree_kernel(const int64_t* indptr, const int32_t* idx,
+                                                     const uint32_t* cnt, int64_t n, const double* w,
+                                                     const double* s, const double* c, double* dinv,
+                                                     double* r64, float* r32, int32_t* status) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (row >= n) return;
+  const int64_t e0 = indptr[row], e1 = indptr[row + 1];
+  double acc = 0.0;
+  for (int64_t e = e0 + lane; e < e1; e += 64) {
+    const int32_t j = idx[e];
+    acc = fma(val64<FV>(cnt, e) * w[j], c[j], acc);
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0) {
+    const double d = acc * s[row] - 1.0;
+    const double di = 1.0 / d;
+    dinv[row] = di;
+    double rr = 0.0;
+    if (e1 > e0) {
+      if (!(d > 0.0)) {
+        if (atomicCAS(&status[0], 0, 1) == 0) {
+          status[1] = (int32_t)(row & 0xFFFFFFFF);
+          status[2] = (int32_t)(row >> 32);
+        }
+      } else {
+        rr = s[row] * sqrt(di);
+      }
+    }
+    r64[row] = rr;
+    r32[row] = (float)rr;
+  }
+}
+
+__global__ void square_f32_kernel(const double* w, int64_t m, float* w2) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) w2[j] = (float)(w[j] * w[j]);
+}
+
+// ---------------------------------------------------------------------------
+// Operator application (FP32 sparse products)
+// ---------------------------------------------------------------------------
+// Vs[i][c] = r_i * V[i][c] as FP32, columns >= b zero-filled.
+__global__ void scale_in_kernel(const double* V, int64_t ldv, int64_t n, int b, const float* r,
+                                float* Vs) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t >> 5;
+  const int c = t & 31;
+  if (i >= n) return;
+  Vs[i * kB + c] = c < b ? (float)((double)r[i] * V[i * ldv + c]) : 0.f;
+}
+
+// out[row][lane] = post(sum_{e in row} cnt[e] * dense[col[e]][lane]) for 32 lanes
+// per row (two rows per wave). Fixed summation order (entry order).
+template <int EPI, bool FV>
+__global__ void __launch_bounds__(256) spmm32_kernel(const int64_t* __restrict__ ptr,
+                                                     const int32_t* __restrict__ col,
+                                                     const uint32_t* __restrict__ cnt, int64_t nrows,
+                                                     const float* __restrict__ dense,
+                                                     const float* __restrict__ rowscale,
+                                                     float* __restrict__ out32,
+                                                     // EPI 1: FP64 output with the -dinv*V term
+                                                     const double* __restrict__ V, int64_t ldv,
+                                                     const double* __restrict__ dinv, double* __restrict__ Y,
+                                                     int64_t ldy, int b) {
+  const int lane = threadIdx.x & 63;
+  const int half = lane >> 5;
+  const int c = lane & 31;
+  const int64_t row = ((((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) << 1) + half;
+  const bool active = row < nrows;
+  int64_t e0 = 0, e1 = 0;
+  if (active) {
+    e0 = ptr[row];
+    e1 = ptr[row + 1];
+  }
+  float acc = 0.f;
+  for (int64_t base = e0; base < e1; base += 32) {
+    const int64_t e = base + c;
+    int32_t j = 0;
+    float v = 0.f;
+    if (e < e1) {
+      j = col[e];
+      v = val32<FV>(cnt, e);
+    }
+    const int nb = (int)min((int64_t)32, e1 - base);
+    for (int t = 0; t < nb; ++t) {
+      const int32_t jt = __shfl(j, t, 32);
+      const float vt = __shfl(v, t, 32);
+      acc = fmaf(vt, dense[(int64_t)jt * kB + c], acc);
+    }
+  }
+  if (!active) return;
+  if (EPI == 0) {
+    out32[row * kB + c] = acc * rowscale[row];
+  } else if (c < b) {
+    Y[row * ldy + c] = (double)(acc * rowscale[row]) - dinv[row] * V[row * ldv + c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 tall-skinny products for the block Lanczos basis
+// ---------------------------------------------------------------------------
+constexpr int kTnRows = 32;
+// partial[chunk][a][c] = sum_{i in chunk} A[i][a0 + a] * B[i][c]  (a < 32, c < kb <= 32)
+__global__ void __launch_bounds__(256) gemm_tn_kernel(const double* A, int64_t lda, const double* B,
+                                                      int64_t ldb, int64_t n, int ka, int kb,
+                                                      int64_t rows_per_chunk, double* partial) {
+  __shared__ double As[kTnRows][33];
+  __shared__ double Bs[kTnRows][33];
+  const int64_t chunk = blockIdx.x;
+  const int a0 = blockIdx.y * 32;
+  const int t = threadIdx.x;
+  const int oa = t >> 3;        // 0..31 output row (a)
+  const int oc = (t & 7) * 4;   // 4 output columns
+  double acc[4] = {0, 0, 0, 0};
+  const int64_t r0 = chunk * rows_per_chunk, r1 = min(n, r0 + rows_per_chunk);
+  for (int64_t rb = r0; rb < r1; rb += kTnRows) {
+    for (int q = t; q < kTnRows * 32; q += 256) {
+      const int rr = q >> 5, cc = q & 31;
+      const int64_t i = rb + rr;
+      As[rr][cc] = (i < r1 && a0 + cc < ka) ? A[i * lda + a0 + cc] : 0.0;
+      Bs[rr][cc] = (i < r1 && cc < kb) ? B[i * ldb + cc] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int rr = 0; rr < kTnRows; ++rr) {
+      const double av = As[rr][oa];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = fma(av, Bs[rr][oc + q], acc[q]);
+    }
+    __syncthreads();
+  }
+  double* P = partial + chunk * (int64_t)ka * kb;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (a0 + oa < ka && oc + q < kb) P[(int64_t)(a0 + oa) * kb + oc + q] = acc[q];
+}
+
+__global__ void reduce_partials_kernel(const double* partial, int64_t nchunks, int64_t count, double* C) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  double s = 0.0;
+  for (int64_t q = 0; q < nchunks; ++q) s += partial[q * count + i];
+  C[i] = s;
+}
+
+// C[i][c] = beta*C[i][c] + alpha * sum_a A[i][a] B[a][c]; block = 8 rows x 32 columns.
+__global__ void __launch_bounds__(256) gemm_nn_kernel(const double* A, int64_t lda, const double* B,
+                                                      int64_t n, int ka, int kb, double alpha,
+                                                      double beta, double* C, int64_t ldc) {
+  __shared__ double Bs[32][33];
+  const int c = threadIdx.x & 31;
+  const int64_t i = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);
+  double acc = 0.0;
+  for (int a0 = 0; a0 < ka; a0 += 32) {
+    for (int q = threadIdx.x; q < 32 * 32; q += 256) {
+      const int ra = q >> 5, cc = q & 31;
+      Bs[ra][cc] = (a0 + ra < ka && cc < kb) ? B[(int64_t)(a0 + ra) * kb + cc] : 0.0;
+    }
+    __syncthreads();
+    if (i < n) {
+      const int lim = min(32, ka - a0);
+      for (int ra = 0; ra < lim; ++ra) acc = fma(A[i * lda + a0 + ra], Bs[ra][c], acc);
+    }
+    __syncthreads();
+  }
+  if (i < n && c < kb) {
+    const double prev = beta == 0.0 ? 0.0 : beta * C[i * ldc + c];
+    C[i * ldc + c] = prev + alpha * acc;
+  }
+}
+
+__global__ void random_kernel(uint64_t seed, int64_t n, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t z = seed * 0x9E3779B97F4A7C15ull + (uint64_t)i * 0xD1B54A32D192ED03ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  out[i] = ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0) - 0.5;
+}
+
+}  // namespace spectral
+}  // namespace snapgpu
+
+using namespace snapgpu;
+using namespace snapgpu::spectral;
+
+static inline unsigned grid_for(int64_t work, int per_block, int64_t cap = 1 << 20) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, per_block), cap));
+}
+
+extern "C" {
+
+int snapgpu_csr_select_count(const int64_t* d_indptr, const void* d_idx, int32_t index_bits,
+                             int64_t n, const int32_t* d_col_map, int64_t* d_row_nnz, void* stream) {
+  clear_error();
+  SG_ARG(index_bits == 32 || index_bits == 64, "index_bits must be 32 or 64");
+  if (n == 0) return SNAPGPU_OK;
+  const unsigned g = (unsigned)ceil_div(n, 4);
+  if (index_bits == 32)
+    select_count_kernel<int32_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int32_t*)d_idx, n, d_col_map, d_row_nnz);
+  else
+    select_count_kernel<int64_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int64_t*)d_idx, n, d_col_map, d_row_nnz);
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
+int snapgpu_csr_select_fill(const int64_t* d_indptr, const void* d_idx, int32_t index_bits,
+                            const uint32_t* d_val, int64_t n, const int32_t* d_col_map,
+                            const int64_t* d_new_indptr, int32_t* d_new_idx, uint32_t* d_new_val,
+                            void* stream) {
+  clear_error();
+  SG_ARG(index_bits == 32 || index_bits == 64, "index_bits must be 32 or 64");
+  if (n == 0) return SNAPGPU_OK;
+  const unsigned g = (unsigned)ceil_div(n, 4);
+  if (index_bits == 32)
+    select_fill_kernel<int32_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int32_t*)d_idx, d_val, n, d_col_map, d_new_indptr, d_new_idx, d_new_val);
+  else
+    select_fill_kernel<int64_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int64_t*)d_idx, d_val, n, d_col_map, d_new_indptr, d_new_idx, d_new_val);
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
+int64_t snapgpu_csr_transpose_workspace_bytes(int64_t m, int64_t chunk_nnz) {
+  size_t tmp = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (int)std::min<int64_t>(chunk_nnz, INT32_MAX));
+  return (int64_t)m * 8 + 256 + chunk_nnz * 4 * 4 + 4 * 256 + (int64_t)tmp + 256;
+}
+
+// Stable transpose. colptr[m+1] out; rows ascending inside each column.
+// chunk_nnz bounds the entries sorted at once (whole rows per chunk).
+int snapgpu_csr_transpose(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_val,
+                          int64_t n, int64_t m, int64_t nnz, const int64_t* h_indptr,
+                          int64_t chunk_nnz, int64_t* d_colptr, int32_t* d_rowidx,
+                          uint32_t* d_cval, void* d_ws, int64_t ws_bytes, void* stream) {
+  clear_error();
+  SG_ARG(n >= 0 && m >= 1 && nnz >= 0, "bad shape");
+  SG_ARG(m < (1ll << 32) && n < (1ll << 32), "shape too large for 32-bit ids");
+  SG_ARG(chunk_nnz >= 1 && chunk_nnz <= INT32_MAX, "bad chunk size");
+  SG_ARG(ws_bytes >= snapgpu_csr_transpose_workspace_bytes(m, chunk_nnz), "workspace too small");
+  SG_ARG(h_indptr != nullptr, "host copy of indptr required");
+  hipStream_t st = as_stream(stream);
+  char* base = (char*)(((uintptr_t)d_ws + 255) & ~(uintptr_t)255);
+  int64_t* cursor = (int64_t*)base;
+  char* p = base + (m * 8 + 255) / 256 * 256;
+  auto take = [&](int64_t bytes) { char* q = p; p += (bytes + 255) / 256 * 256; return q; };
+  uint32_t* keys_in = (uint32_t*)take(chunk_nnz * 4);
+  uint32_t* keys_out = (uint32_t*)take(chunk_nnz * 4);
+  uint32_t* pos_in = (uint32_t*)take(chunk_nnz * 4);
+  uint32_t* pos_out = (uint32_t*)take(chunk_nnz * 4);
+  void* tmp = p;
+  size_t tmp_bytes = (size_t)(ws_bytes - (p - (char*)d_ws));
+  // column counts -> colptr
+  SG_HIP(hipMemsetAsync(d_colptr, 0, (m + 1) * 8, st));
+  if (nnz > 0) {
+    col_count_kernel<<<grid_for(nnz, 256, 16384), 256, 0, st>>>(d_idx, nnz, (unsigned long long*)(d_colptr + 1));
+    SG_LAUNCH_CHECK();
+  }
+  const int64_t sws = snapgpu_scan_workspace_bytes(m);
+  SG_ARG((int64_t)tmp_bytes >= sws, "workspace too small for the scan");
+  int rc = snapgpu_scan_inclusive_i64(d_colptr + 1, m, tmp, sws, stream);
+  if (rc) return rc;
+  SG_HIP(hipMemcpyAsync(cursor, d_colptr, m * 8, hipMemcpyDeviceToDevice, st));
+  int end_bit = 1;
+  while (end_bit < 32 && (1ll << end_bit) < m) ++end_bit;
+  int64_t r0 = 0;
+  while (r0 < n) {
+    // whole rows, at most chunk_nnz entries (a single longer row is an error)
+    int64_t lo = r0 + 1, hi = n;
+    if (h_indptr[r0 + 1] - h_indptr[r0] > chunk_nnz) {
+      set_error("a row has more entries than the transpose chunk");
+      return SNAPGPU_ERR_CAPACITY;
+    }
+    while (lo < hi) {  // last r1 with indptr[r1] - indptr[r0] <= chunk_nnz
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (h_indptr[mid] - h_indptr[r0] <= chunk_nnz) lo = mid; else hi = mid - 1;
+    }
+    const int64_t r1 = lo;
+    const int64_t e0 = h_indptr[r0], cn = h_indptr[r1] - e0;
+    if (cn > 0) {
+      SG_HIP(hipMemcpyAsync(keys_in, d_idx + e0, cn * 4, hipMemcpyDeviceToDevice, st));
+      iota_kernel<<<grid_for(cn, 256, 16384), 256, 0, st>>>(pos_in, cn);
+      SG_LAUNCH_CHECK();
+      SG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, keys_out, pos_in, pos_out,
+                                                (int)cn, 0, end_bit, st));
+      // keys_in is free again: reuse it for the row ids of the chunk's entries
+      entry_rows_kernel<<<(unsigned)ceil_div(r1 - r0, 4), 256, 0, st>>>(d_indptr, r0, r1, keys_in);
+      SG_LAUNCH_CHECK();
+      scatter_chunk_kernel<<<grid_for(cn, 256, 16384), 256, 0, st>>>(keys_out, pos_out, cn, keys_in,
+                                                                     d_val + e0, cursor, d_rowidx, d_cval);
+      SG_LAUNCH_CHECK();
+      cursor_advance_kernel<<<grid_for(cn, 256, 16384), 256, 0, st>>>(keys_out, cn, cursor);
+      SG_LAUNCH_CHECK();
+    }
+    r0 = r1;
+  }
+  return SNAPGPU_OK;
+}
+
+int snapgpu_spectral_idf(const int64_t* d_colptr, int64_t m, int64_t n, double* d_w, void* d_ws16,
+                         void* stream) {
+  clear_error();
+  SG_ARG(m >= 1 && n >= 1, "bad shape");
+  hipStream_t st = as_stream(stream);
+  unsigned long long init[2] = {~0ull, 0ull};
+  SG_HIP(hipMemcpyAsync(d_ws16, init, 16, hipMemcpyHostToDevice, st));
+  df_minmax_kernel<<<grid_for(m, 256, 1024), 256, 0, st>>>(d_colptr, m, (unsigned long long*)d_ws16);
+  SG_LAUNCH_CHECK();
+  idf_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(d_colptr, m, n, (const unsigned long long*)d_ws16, d_w);
+  SG_LAUNCH_CHECK();
+  SG_HIP(hipStreamSynchronize(st));  // `init` lives on this stack frame
+  return SNAPGPU_OK;
+}
+
+int snapgpu_spectral_prepare(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_cnt,
+                             int64_t n, const int64_t* d_colptr, const int32_t* d_rowidx,
+                             const uint32_t* d_ccnt, int64_t m, const double* d_w, double* d_s,
+                             double* d_c, double* d_dinv, double* d_r64, float* d_r32, float* d_w2,
+                             int32_t value_type, int32_t* d_status, void* stream) {
+  clear_error();
+  SG_ARG(n >= 1 && m >= 1, "bad shape");
+  SG_ARG(value_type == 0 || value_type == 1, "value_type must be 0 (uint32) or 1 (float32)");
+  hipStream_t st = as_stream(stream);
+  SG_HIP(hipMemsetAsync(d_status, 0, 8 * sizeof(int32_t), st));
+  const unsigned gn = (unsigned)ceil_div(n, 4), gm = (unsigned)ceil_div(m, 4);
+  if (value_type == 0) {
+    row_scale_kernel<false><<<gn, 256, 0, st>>>(d_indptr, d_idx, d_cnt, n, d_w, d_s);
+    colsum_kernel<false><<<gm, 256, 0, st>>>(d_colptr, d_rowidx, d_ccnt, m, d_w, d_s, d_c);
+    degree_kernel<false><<<gn, 256, 0, st>>>(d_indptr, d_idx, d_cnt, n, d_w, d_s, d_c, d_dinv, d_r64,
+                                              d_r32, d_status);
+  } else {
+    row_scale_kernel<true><<<gn, 256, 0, st>>>(d_indptr, d_idx, d_cnt, n, d_w, d_s);
+    colsum_kernel<true><<<gm, 256, 0, st>>>(d_colptr, d_rowidx, d_ccnt, m, d_w, d_s, d_c);
+    degree_kernel<true><<<gn, 256, 0, st>>>(d_indptr, d_idx, d_cnt, n, d_w, d_s, d_c, d_dinv, d_r64,
+                                             d_r32, d_status);
+  }
+  SG_LAUNCH_CHECK();
+  square_f32_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(d_w, m, d_w2);
+  SG_LAUNCH_CHECK();
+  int32_t h[8];
+  SG_HIP(hipMemcpyAsync(h, d_status, sizeof(h), hipMemcpyDeviceToHost, st));
+  SG_HIP(hipStreamSynchronize(st));
+  if (h[0] != 0) {
+    const int64_t row = (int64_t)(uint32_t)h[1] | ((int64_t)h[2] << 32);
+    set_error("cell " + std::to_string(row) +
+              " shares no selected feature with any other cell (degree <= 0)");
+    return SNAPGPU_ERR_DATA;
+  }
+  return SNAPGPU_OK;
+}
+
+int64_t snapgpu_spectral_apply_workspace_bytes(int64_t n, int64_t m) {
+  return (n * kB + m * kB) * 4 + 512;
+}
+
+int snapgpu_spectral_apply(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_cnt,
+                           int64_t n, const int64_t* d_colptr, const int32_t* d_rowidx,
+                           const uint32_t* d_ccnt, int64_t m, const float* d_r32, const float* d_w2,
+                           const double* d_dinv, const double* d_V, int64_t ldv, int32_t b,
+                           double* d_Y, int64_t ldy, int32_t value_type, void* d_ws, int64_t ws_bytes,
+                           void* stream) {
+  clear_error();
+  SG_ARG(b >= 1 && b <= kB, "block width must be in [1, 32]");
+  SG_ARG(value_type == 0 || value_type == 1, "value_type must be 0 (uint32) or 1 (float32)");
+  SG_ARG(ldv >= b && ldy >= b, "leading dimension < b");
+  SG_ARG(ws_bytes >= snapgpu_spectral_apply_workspace_bytes(n, m), "workspace too small");
+  hipStream_t st = as_stream(stream);
+  float* Vs = (float*)(((uintptr_t)d_ws + 255) & ~(uintptr_t)255);
+  float* Z = Vs + n * kB;
+  scale_in_kernel<<<(unsigned)ceil_div(n * 32, 256), 256, 0, st>>>(d_V, ldv, n, b, d_r32, Vs);
+  SG_LAUNCH_CHECK();
+  // Z = w^2 o (X^T Vs)   (CSC rows = features);  Y = r o (X Z) - dinv o V
+  const unsigned gm = (unsigned)ceil_div(m, 8), gn = (unsigned)ceil_div(n, 8);
+  if (value_type == 0) {
+    spmm32_kernel<0, false><<<gm, 256, 0, st>>>(d_colptr, d_rowidx, d_ccnt, m, Vs, d_w2, Z, nullptr, 0,
+                                                nullptr, nullptr, 0, b);
+    spmm32_kernel<1, false><<<gn, 256, 0, st>>>(d_indptr, d_idx, d_cnt, n, Z, d_r32, nullptr, d_V, ldv,
+                                                d_dinv, d_Y, ldy, b);
+  } else {
+    spmm32_kernel<0, true><<<gm, 256, 0, st>>>(d_colptr, d_rowidx, d_ccnt, m, Vs, d_w2, Z, nullptr, 0,
+                                               nullptr, nullptr, 0, b);
+    spmm32_kernel<1, true><<<gn, 256, 0, st>>>(d_indptr, d_idx, d_cnt, n, Z, d_r32, nullptr, d_V, ldv,
+                                               d_dinv, d_Y, ldy, b);
+  }
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
+int64_t snapgpu_gemm_tn_workspace_bytes(int64_t n, int32_t ka, int32_t kb) {
+  const int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(512, ceil_div(n, 4096)));
+  return chunks * (int64_t)ka * kb * 8 + 256;
+}
+
+// C[ka x kb] = A[n x ka]^T B[n x kb] (row-major, leading dims lda/ldb).
+int snapgpu_gemm_tn_f64(const double* d_A, int64_t lda, const double* d_B, int64_t ldb, int64_t n,
+                        int32_t ka, int32_t kb, double* d_C, void* d_ws, int64_t ws_bytes,
+                        void* stream) {
+  clear_error();
+  SG_ARG(ka >= 1 && kb >= 1 && kb <= 32, "need 1 <= kb <= 32");
+  SG_ARG(ws_bytes >= snapgpu_gemm_tn_workspace_bytes(n, ka, kb), "workspace too small");
+  hipStream_t st = as_stream(stream);
+  const int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(512, ceil_div(n, 4096)));
+  const int64_t rpc = ceil_div(std::max<int64_t>(n, 1), chunks);
+  double* part = (double*)d_ws;
+  gemm_tn_kernel<<<dim3((unsigned)chunks, (unsigned)ceil_div(ka, 32)), 256, 0, st>>>(
+      d_A, lda, d_B, ldb, n, ka, kb, rpc, part);
+  SG_LAUNCH_CHECK();
+  const int64_t count = (int64_t)ka * kb;
+  reduce_partials_kernel<<<(unsigned)ceil_div(count, 256), 256, 0, st>>>(part, chunks, count, d_C);
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
+// C[n x kb] = beta*C + alpha * A[n x ka] B[ka x kb] (B row-major, contiguous).
+int snapgpu_gemm_nn_f64(const double* d_A, int64_t lda, const double* d_B, int64_t n, int32_t ka,
+                        int32_t kb, double alpha, double beta, double* d_C, int64_t ldc,
+                        void* stream) {
+  clear_error();
+  SG_ARG(ka >= 1 && kb >= 1 && kb <= 32, "need 1 <= kb <= 32");
+  if (n == 0) return SNAPGPU_OK;
+  gemm_nn_kernel<<<(unsigned)ceil_div(n, 8), 256, 0, as_stream(stream)>>>(d_A, lda, d_B, n, ka, kb,
+                                                                          alpha, beta, d_C, ldc);
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
+int snapgpu_random_f64(uint64_t seed, int64_t count, double* d_out, void* stream) {
+  clear_error();
+  if (count <= 0) return SNAPGPU_OK;
+  random_kernel<<<(unsigned)ceil_div(count, 256), 256, 0, as_stream(stream)>>>(seed, count, d_out);
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
+}  // extern "C"

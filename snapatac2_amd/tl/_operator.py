@@ -1,0 +1,150 @@
+"""Device state of the normalized-cosine operator used by spectral().
+
+A = D^-1/2 (X~ X~^T - I) D^-1/2 with X~ = rownorm(X diag(w)) and
+D = diag(X~ X~^T 1 - 1) (snapatac2-python/src/embedding.rs:131-171, 269-326).
+The matrix stays as counts in CSR + CSC; see csrc/spectral.hip.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .. import _lib
+from .._device import DeviceCSR
+
+TRANSPOSE_CHUNK = 1 << 28  # entries sorted per transpose chunk
+
+
+def _to_device_csr(X, torch) -> DeviceCSR:
+    """A scipy/numpy matrix (or DeviceCSR) as a DeviceCSR with int32 columns."""
+    if isinstance(X, DeviceCSR):
+        return X
+    import scipy.sparse as sp
+    Xc = sp.csr_matrix(X)
+    Xc.sort_indices()
+    data = Xc.data
+    if data.dtype.kind in "ui" and (data.size == 0 or data.min() >= 0) and \
+            (data.size == 0 or data.max() < 2**32):
+        vals = data.astype(np.uint32).view(np.int32)
+        dt = np.uint32
+    else:
+        vals = data.astype(np.float32).view(np.int32)
+        dt = np.float32
+    dev = torch.device("cuda")
+    return DeviceCSR(torch.from_numpy(Xc.indptr.astype(np.int64)).to(dev),
+                     torch.from_numpy(Xc.indices.astype(np.int32)).to(dev),
+                     torch.from_numpy(vals).to(dev), Xc.shape, dt)
+
+
+def select_columns(X: DeviceCSR, cols: np.ndarray | None) -> DeviceCSR:
+    """Columns `cols` (ascending original ids) of X, on the device."""
+    torch = _lib.require_gpu()
+    if cols is None:
+        if X.indices.dtype == torch.int32:
+            return X
+        cols = np.arange(X.shape[1])
+    L = _lib.lib()
+    n, m_all = X.shape
+    col_map = np.full(m_all, -1, dtype=np.int32)
+    col_map[cols] = np.arange(len(cols), dtype=np.int32)
+    cm = torch.from_numpy(col_map).to(X.device)
+    bits = 32 if X.indices.dtype == torch.int32 else 64
+    indptr = torch.zeros(n + 1, dtype=torch.int64, device=X.device)
+    st = _lib.stream_ptr()
+    _lib.check(L.snapgpu_csr_select_count(X.indptr.data_ptr(), X.indices.data_ptr(), bits, n,
+                                          cm.data_ptr(), indptr.data_ptr() + 8, st))
+    ws_b = int(L.snapgpu_scan_workspace_bytes(n))
+    ws = torch.empty(max(ws_b, 8), dtype=torch.uint8, device=X.device)
+    _lib.check(L.snapgpu_scan_inclusive_i64(indptr.data_ptr() + 8, n, ws.data_ptr(), ws_b, st))
+    nnz = int(indptr[-1].item())
+    idx = torch.empty(nnz, dtype=torch.int32, device=X.device)
+    val = torch.empty(nnz, dtype=torch.int32, device=X.device)
+    _lib.check(L.snapgpu_csr_select_fill(X.indptr.data_ptr(), X.indices.data_ptr(), bits,
+                                         X.data.data_ptr(), n, cm.data_ptr(), indptr.data_ptr(),
+                                         idx.data_ptr(), val.data_ptr(), st))
+    return DeviceCSR(indptr, idx, val, (n, len(cols)), X.data_dtype)
+
+
+class SpectralOperator:
+    """Holds CSR/CSC counts, weights, scales and degrees of A on the device."""
+
+    def __init__(self, X: DeviceCSR, feature_weights=None):
+        torch = _lib.require_gpu()
+        L = _lib.lib()
+        self.torch = torch
+        self.X = X
+        n, m = X.shape
+        self.n, self.m = n, m
+        dev = X.device
+        self.value_type = 1 if np.dtype(X.data_dtype) == np.float32 else 0
+        st = _lib.stream_ptr()
+        # CSC copy (deterministic stable transpose)
+        h_indptr = X.indptr.cpu().numpy()
+        chunk = int(min(max(X.nnz, 1), TRANSPOSE_CHUNK))
+        ws_b = int(L.snapgpu_csr_transpose_workspace_bytes(m, chunk))
+        ws = torch.empty(ws_b, dtype=torch.uint8, device=dev)
+        self.colptr = torch.empty(m + 1, dtype=torch.int64, device=dev)
+        self.rowidx = torch.empty(X.nnz, dtype=torch.int32, device=dev)
+        self.cval = torch.empty(X.nnz, dtype=torch.int32, device=dev)
+        _lib.check(L.snapgpu_csr_transpose(X.indptr.data_ptr(), X.indices.data_ptr(),
+                                           X.data.data_ptr(), n, m, X.nnz, h_indptr.ctypes.data,
+                                           chunk, self.colptr.data_ptr(), self.rowidx.data_ptr(),
+                                           self.cval.data_ptr(), ws.data_ptr(), ws_b, st))
+        del ws
+        if feature_weights is None:
+            self.w = torch.empty(m, dtype=torch.float64, device=dev)
+            scratch = torch.empty(16, dtype=torch.uint8, device=dev)
+            _lib.check(L.snapgpu_spectral_idf(self.colptr.data_ptr(), m, n, self.w.data_ptr(),
+                                              scratch.data_ptr(), st))
+        else:
+            w = np.asarray(feature_weights, dtype=np.float64)
+            if w.shape != (m,):
+                raise ValueError(f"feature_weights has {w.size} entries, expected {m}")
+            self.w = torch.from_numpy(w).to(dev)
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.s = torch.empty(n, **f64)
+        self.c = torch.empty(m, **f64)
+        self.dinv = torch.empty(n, **f64)
+        self.r64 = torch.empty(n, **f64)
+        self.r32 = torch.empty(n, dtype=torch.float32, device=dev)
+        self.w2 = torch.empty(m, dtype=torch.float32, device=dev)
+        status = torch.zeros(8, dtype=torch.int32, device=dev)
+        _lib.check(L.snapgpu_spectral_prepare(
+            X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), n,
+            self.colptr.data_ptr(), self.rowidx.data_ptr(), self.cval.data_ptr(), m,
+            self.w.data_ptr(), self.s.data_ptr(), self.c.data_ptr(), self.dinv.data_ptr(),
+            self.r64.data_ptr(), self.r32.data_ptr(), self.w2.data_ptr(), self.value_type,
+            status.data_ptr(), st))
+        self._ws_b = int(L.snapgpu_spectral_apply_workspace_bytes(n, m))
+        self._ws = torch.empty(self._ws_b, dtype=torch.uint8, device=dev)
+        self.n_applies = 0
+        self.n_columns = 0
+
+    def apply(self, V, Y, b: int | None = None) -> None:
+        """Y[:, :b] = A V[:, :b]; V, Y row-major FP64 device tensors (any row stride)."""
+        L = _lib.lib()
+        X = self.X
+        b = V.shape[1] if b is None else b
+        if b > 32:
+            for c0 in range(0, b, 32):
+                self.apply(V[:, c0:c0 + 32], Y[:, c0:c0 + 32])
+            return
+        _lib.check(L.snapgpu_spectral_apply(
+            X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), self.n,
+            self.colptr.data_ptr(), self.rowidx.data_ptr(), self.cval.data_ptr(), self.m,
+            self.r32.data_ptr(), self.w2.data_ptr(), self.dinv.data_ptr(), V.data_ptr(),
+            V.stride(0), b, Y.data_ptr(), Y.stride(0), self.value_type, self._ws.data_ptr(),
+            self._ws_b, _lib.stream_ptr()))
+        self.n_applies += 1
+        self.n_columns += b
+
+    def matmat(self, V):
+        Y = self.torch.empty_like(V)
+        self.apply(V, Y)
+        return Y
+
+    @property
+    def degree(self):
+        """d = X~ X~^T 1 - 1 (what spectral_mf returns for Nystrom, embedding.rs:189-190)."""
+        return 1.0 / self.dinv

@@ -1,0 +1,133 @@
+"""Path B on the GPU: operator, preparation and eigenpairs vs the oracle.
+
+Tolerances: eigenvalues 1e-5 relative (atol 1e-7), largest principal angle of
+the eigenvector subspace < 1e-4 rad; operator application (FP32 sparse
+products) 1e-5 relative in the max norm; preparation (FP64) 1e-12 relative.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import spectral as osp
+
+pytestmark = pytest.mark.gpu
+
+
+def small_counts(n=300, m=800, density=0.05, seed=0):
+    rng = np.random.default_rng(seed)
+    X = sp.random(n, m, density=density, random_state=rng, format="csr",
+                  data_rvs=lambda k: rng.integers(1, 4, k)).astype(np.uint32)
+    X = X + sp.csr_matrix((np.ones(n, dtype=np.uint32), (np.arange(n), rng.integers(0, 20, n))),
+                          shape=(n, m))
+    return sp.csr_matrix(X, dtype=np.uint32)
+
+
+def _op(torch, X, weights=None):
+    from snapatac2_amd.tl._operator import SpectralOperator, _to_device_csr
+    return SpectralOperator(_to_device_csr(X, torch), weights)
+
+
+def test_transpose_and_prepare(torch_gpu):
+    torch = torch_gpu
+    X = small_counts(2000, 3000, 0.02, seed=1)
+    op = _op(torch, X)
+    Xc = X.tocsc()
+    Xc.sort_indices()
+    assert np.array_equal(op.colptr.cpu().numpy(), Xc.indptr)
+    assert np.array_equal(op.rowidx.cpu().numpy(), Xc.indices)
+    assert np.array_equal(op.cval.cpu().numpy().view(np.uint32), Xc.data)
+    Xh, dinv, w, Xn = osp.operator_parts(X)
+    np.testing.assert_allclose(op.w.cpu().numpy(), w, rtol=1e-14)
+    np.testing.assert_allclose(op.dinv.cpu().numpy(), dinv, rtol=1e-12)
+
+
+def test_operator_application(torch_gpu):
+    torch = torch_gpu
+    X = small_counts(3000, 5000, 0.02, seed=4)
+    op = _op(torch, X)
+    Xh, dinv, _, _ = osp.operator_parts(X)
+    rng = np.random.default_rng(0)
+    for b in (32, 30, 7):
+        V = rng.standard_normal((3000, b))
+        Y = op.matmat(torch.from_numpy(V).cuda()).cpu().numpy()
+        ref = Xh @ (Xh.T @ V) - dinv[:, None] * V
+        err = np.abs(Y - ref).max() / np.abs(ref).max()
+        assert err < 1e-5, (b, err)
+
+
+def _compare(evals, evecs, oe, ov, k):
+    np.testing.assert_allclose(evals[:k], oe[:k], rtol=1e-5, atol=1e-7)
+    ang = osp.subspace_angle(evecs[:, :k], ov[:, :k])
+    assert ang < 1e-4, ang
+
+
+@pytest.mark.parametrize("n", [300, 2500])
+def test_spectral_embedding_vs_oracle(torch_gpu, n):
+    from snapatac2_amd.tl._embedding import spectral_embedding
+    import snapatac2_amd as snap
+    X = small_counts(n, 4000, 0.03, seed=n)
+    ad = snap.AnnData(X=X)
+    evals, evecs, st = spectral_embedding(ad, None, 20, 0, return_stats=True)
+    oe, ov = osp.spectral_embedding(X, None, 20, 0)
+    _compare(evals, evecs, oe, ov, 20)
+    print("stats", st)
+
+
+def test_synthetic_2k_cells_pipeline(torch_gpu):
+    import snapatac2_amd as snap
+    from snapatac2_amd import synth
+    ad = synth.synthetic_anndata(2000, seed=0)
+    snap.pp.add_tile_matrix(ad)
+    snap.pp.select_features(ad, n_features=50000)
+    from snapatac2_amd.tl._embedding import spectral_embedding
+    evals, evecs, st = spectral_embedding(ad, ad.var["selected"].to_numpy(), 30, 0,
+                                          return_stats=True)
+    X = ad.X.to_scipy()
+    oe, ov = osp.spectral_embedding(X, ad.var["selected"].to_numpy(), 30, 0)
+    print("stats", st, "evals", evals[:5], evals[-3:], "oracle", oe[-3:])
+    _compare(evals, evecs, oe, ov, 30)
+    # drop-in: weighted_by_sd keeps positive eigenvalues, scales by sqrt
+    snap.tl.spectral(ad, n_comps=30)
+    E = ad.obsm["X_spectral"]
+    lam = ad.uns["spectral_eigenvalue"]
+    assert E.shape == (2000, lam.size) and np.all(lam > 0)
+    np.testing.assert_allclose(np.linalg.norm(E, axis=0) ** 2, lam, rtol=1e-8)
+
+
+def test_determinism_and_dense_float_input(torch_gpu):
+    # reference test_reproducibility (test_tools.py:95-122): 50 x 100 dense floats in [1, 100]
+    import snapatac2_amd as snap
+    rng = np.random.default_rng(0)
+    M = rng.uniform(1, 100, (50, 100))
+    ad = snap.AnnData(X=sp.csr_matrix(M))
+    outs = [snap.tl.spectral(ad, features=None, random_state=0, inplace=False)[1] for _ in range(3)]
+    for o in outs:
+        assert np.array_equal(o, outs[0])
+    oe, ov = osp.spectral(sp.csr_matrix(M), 30, None, 0)
+    ge, gv = snap.tl.spectral(ad, features=None, random_state=0, inplace=False)
+    np.testing.assert_allclose(ge, oe, rtol=1e-5, atol=1e-7)
+    snap.tl.spectral(ad, features=None, random_state=0)
+    knns = [snap.pp.knn(ad, n_neighbors=25, inplace=False).toarray() for _ in range(3)]
+    for k in knns:
+        assert np.array_equal(k, knns[0])
+
+
+def test_block_lanczos_determinism(torch_gpu):
+    from snapatac2_amd.tl._embedding import spectral_embedding
+    import snapatac2_amd as snap
+    X = small_counts(3000, 4000, 0.03, seed=9)
+    ad = snap.AnnData(X=X)
+    a = spectral_embedding(ad, None, 30, 0)
+    b = spectral_embedding(ad, None, 30, 0)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_errors(torch_gpu):
+    import snapatac2_amd as snap
+    ad = snap.AnnData(X=small_counts(100, 200))
+    with pytest.raises(NameError):
+        snap.tl.spectral(ad)
+    with pytest.raises(ValueError):
+        snap.tl.spectral(ad, features=None, sample_size=1)
+    with pytest.raises(NotImplementedError):
+        snap.tl.spectral(ad, features=None, sample_size=0.5)

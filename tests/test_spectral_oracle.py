@@ -1,0 +1,79 @@
+"""Pins the path-B oracle (oracle/spectral.py) without a GPU."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import spectral as osp
+
+
+def small_counts(n=300, m=800, density=0.05, seed=0):
+    rng = np.random.default_rng(seed)
+    X = sp.random(n, m, density=density, random_state=rng, format="csr",
+                  data_rvs=lambda k: rng.integers(1, 4, k)).astype(np.uint32)
+    # make sure every row shares features with others
+    X = X + sp.csr_matrix((np.ones(n, dtype=np.uint32), (np.arange(n), rng.integers(0, 20, n))),
+                          shape=(n, m))
+    return sp.csr_matrix(X, dtype=np.uint32)
+
+
+def test_idf_special_cases():
+    # df: col0 in all 4 rows (-> n-1), col1 in 1 row, col2 in none (-> 1), col3 in 2
+    X = sp.csr_matrix(np.array([[1, 1, 0, 1], [2, 0, 0, 0], [1, 0, 0, 3], [5, 0, 0, 0]]))
+    w = osp.idf(X)
+    np.testing.assert_allclose(w, [np.log(4 / 3), np.log(4), np.log(4), np.log(2)])
+    # all-equal document frequencies -> weights 1 (embedding.rs:273-274)
+    Y = sp.csr_matrix(np.array([[1, 2], [3, 4], [5, 6.0]]))
+    assert np.array_equal(osp.idf(Y), [1.0, 1.0])
+
+
+def test_hand_derived_operator_entries():
+    # A_ij = (S - I)_ij / sqrt(d_i d_j), S = cosine similarity with weights w^2
+    # (values multiplied by w, then rows L2-normalised: embedding.rs:314-326)
+    X = np.array([[1.0, 2.0, 0.0], [0.0, 1.0, 1.0], [3.0, 0.0, 1.0], [1.0, 1.0, 1.0]])
+    w = np.array([0.5, 2.0, 1.5])
+    n = X.shape[0]
+    S = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            num = sum(X[i, k] * X[j, k] * w[k] * w[k] for k in range(3))
+            ni = np.sqrt(sum((X[i, k] * w[k]) ** 2 for k in range(3)))
+            nj = np.sqrt(sum((X[j, k] * w[k]) ** 2 for k in range(3)))
+            S[i, j] = num / (ni * nj)
+    d = S.sum(axis=1) - 1.0
+    A = (S - np.eye(n)) / np.sqrt(np.outer(d, d))
+    np.testing.assert_allclose(osp.dense_operator(sp.csr_matrix(X), feature_weights=w), A,
+                               rtol=1e-13, atol=1e-15)
+
+
+def test_known_eigenpair():
+    X = small_counts()
+    Xh, dinv, w, Xn = osp.operator_parts(X)
+    d = 1.0 / dinv
+    u = np.sqrt(d)
+    Au = Xh @ (u @ Xh) - dinv * u
+    np.testing.assert_allclose(Au, u, rtol=1e-10)
+    evals, evecs = osp.spectral_embedding(X, None, 5)
+    assert abs(evals[0] - 1.0) < 1e-10
+    assert osp.subspace_angle(evecs[:, :1], u[:, None]) < 1e-6
+
+
+def test_eigsh_matches_dense_eigh():
+    X = small_counts(seed=2)
+    A = osp.dense_operator(X)
+    lam, U = np.linalg.eigh(A)
+    k = 12
+    order = np.argsort(-np.abs(lam))[:k]
+    evals, evecs = osp.spectral_embedding(X, None, k)
+    np.testing.assert_allclose(np.sort(evals), np.sort(lam[order]), rtol=1e-9, atol=1e-12)
+    assert osp.subspace_angle(evecs, U[:, order]) < 1e-6
+
+
+def test_features_mask_and_determinism():
+    X = small_counts(seed=3)
+    mask = np.zeros(X.shape[1], dtype=bool)
+    mask[::2] = True
+    a = osp.spectral(X, 8, features=mask)
+    b = osp.spectral(X, 8, features=mask)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    c = osp.spectral(X[:, mask], 8)
+    np.testing.assert_allclose(a[0], c[0], rtol=1e-12)
