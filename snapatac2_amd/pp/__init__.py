@@ -1,5 +1,6 @@
 """Preprocessing drop-ins (snapatac2.pp)."""
-from ._basic import add_tile_matrix, select_features, make_tile_matrix
+from ._basic import add_tile_matrix, make_tile_matrix, make_tile_matrix_chunked, select_features
 from ._knn import knn
 
-__all__ = ["add_tile_matrix", "select_features", "make_tile_matrix", "knn"]
+__all__ = ["add_tile_matrix", "select_features", "make_tile_matrix", "make_tile_matrix_chunked",
+           "knn"]

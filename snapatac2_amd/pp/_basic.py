@@ -108,6 +108,61 @@ def make_tile_matrix(
     return DeviceCSR(indptr, out_idx, out_val, (n, n_bins)), BinIndex(kept, bin_size)
 
 
+def make_tile_matrix_chunked(chunks, chrom_sizes: dict[str, int], bin_size: int = 500,
+                             exclude_chroms: list[str] | None = None,
+                             min_frag_size: int | None = None, max_frag_size: int | None = None,
+                             counting_strategy: str = "paired-insertion") -> tuple[DeviceCSR, BinIndex]:
+    """Tile matrix of fragments given as consecutive chunks of whole cells
+    [(first_cell, DeviceFragments), ...] written into one CSR: a count pass over
+    every chunk, one scan, then a fill pass per chunk at the global offsets."""
+    torch = _lib.require_gpu()
+    L = _lib.lib()
+    names = list(chrom_sizes.keys())
+    lens = np.asarray([chrom_sizes[c] for c in names], dtype=np.uint64)
+    excl = set(exclude_chroms or []) & set(names)
+    excluded = np.asarray([c in excl for c in names], dtype=np.uint8)
+    n_chrom = len(names)
+    table_h = np.zeros(L.snapgpu_tile_table_bytes(n_chrom), dtype=np.uint8)
+    nb = C.c_uint64(0)
+    _lib.check(L.snapgpu_tile_table_pack(lens.ctypes.data, excluded.ctypes.data, n_chrom,
+                                         int(bin_size), table_h.ctypes.data, C.byref(nb)))
+    n_bins = int(nb.value)
+    dev = chunks[0][1].device
+    table = torch.from_numpy(table_h).to(dev)
+    n = sum(f.shape[0] for _, f in chunks)
+    strategy = _STRATEGY[counting_strategy] if chunks[0][1].paired else _SINGLE
+    min_len = 0 if min_frag_size is None else int(min_frag_size)
+    max_len = 0xFFFFFFFF if max_frag_size is None else int(max_frag_size)
+    indptr = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+    status = torch.zeros(8, dtype=torch.int32, device=dev)
+    stream = _lib.stream_ptr()
+    levels = []
+    for c0, fr in chunks:
+        level = C.c_int32(0)
+        _lib.check(L.snapgpu_tile_count(fr.cell_ptr.data_ptr(), fr.pos.data_ptr(),
+                                        _lib.ptr(fr.values) if fr.paired else 0, fr.shape[0],
+                                        table.data_ptr(), n_chrom, int(bin_size), strategy,
+                                        min_len, max_len, indptr.data_ptr() + 8 * (c0 + 1),
+                                        status.data_ptr(), C.byref(level), stream))
+        levels.append(level.value)
+    ws_b = int(L.snapgpu_scan_workspace_bytes(n))
+    ws = torch.empty(max(ws_b, 8), dtype=torch.uint8, device=dev)
+    _lib.check(L.snapgpu_scan_inclusive_i64(indptr.data_ptr() + 8, n, ws.data_ptr(), ws_b, stream))
+    nnz = int(indptr[-1].item())
+    idx_bits = 32 if n_bins < 2**31 else 64
+    out_idx = torch.empty(nnz, dtype=torch.int32 if idx_bits == 32 else torch.int64, device=dev)
+    out_val = torch.empty(nnz, dtype=torch.int32, device=dev)
+    for (c0, fr), lv in zip(chunks, levels):
+        _lib.check(L.snapgpu_tile_fill(fr.cell_ptr.data_ptr(), fr.pos.data_ptr(),
+                                       _lib.ptr(fr.values) if fr.paired else 0, fr.shape[0],
+                                       table.data_ptr(), n_chrom, int(bin_size), strategy,
+                                       min_len, max_len, lv, indptr.data_ptr() + 8 * c0,
+                                       out_idx.data_ptr(), idx_bits, out_val.data_ptr(),
+                                       status.data_ptr(), stream))
+    kept = {c: chrom_sizes[c] for c in names if c not in excl}
+    return DeviceCSR(indptr, out_idx, out_val, (n, n_bins)), BinIndex(kept, bin_size)
+
+
 def add_tile_matrix(
     adata,
     *,

@@ -120,6 +120,7 @@ class SpectralOperator:
         self._ws = torch.empty(self._ws_b, dtype=torch.uint8, device=dev)
         self.n_applies = 0
         self.n_columns = 0
+        self.events = None  # optional list collecting (start, end) events per application
 
     def apply(self, V, Y, b: int | None = None) -> None:
         """Y[:, :b] = A V[:, :b]; V, Y row-major FP64 device tensors (any row stride)."""
@@ -130,12 +131,19 @@ class SpectralOperator:
             for c0 in range(0, b, 32):
                 self.apply(V[:, c0:c0 + 32], Y[:, c0:c0 + 32])
             return
+        ev = None
+        if self.events is not None:
+            ev = (self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True))
+            ev[0].record()
         _lib.check(L.snapgpu_spectral_apply(
             X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), self.n,
             self.colptr.data_ptr(), self.rowidx.data_ptr(), self.cval.data_ptr(), self.m,
             self.r32.data_ptr(), self.w2.data_ptr(), self.dinv.data_ptr(), V.data_ptr(),
             V.stride(0), b, Y.data_ptr(), Y.stride(0), self.value_type, self._ws.data_ptr(),
             self._ws_b, _lib.stream_ptr()))
+        if ev is not None:
+            ev[1].record()
+            self.events.append(ev)
         self.n_applies += 1
         self.n_columns += b
 
