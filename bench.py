@@ -139,6 +139,7 @@ def cpu_baseline(torch, chunks, cfg, emb, sel_nnz_full):
     cp = fr.cell_ptr[:ns + 1].cpu().numpy()
     pos = fr.pos[:int(cp[-1])].cpu().numpy().view(np.uint32)
     ln = fr.values[:int(cp[-1])].cpu().numpy().view(np.uint32)
+    log(f"[bench] cpu baseline: tile oracle on {ns} cells")
     t0 = time.perf_counter()
     Xh = otile.tile_matrix(cp.astype(np.uint64), pos, ln, cs, 500, ["chrY"])
     t_tile = (time.perf_counter() - t0) / ns * n_cells
@@ -153,8 +154,9 @@ def cpu_baseline(torch, chunks, cfg, emb, sel_nnz_full):
     def f(v):
         matvecs[0] += 1
         return Xhat @ (v.T @ Xhat).T - dinv * v
+    log("[bench] cpu baseline: scipy eigsh on the sample")
     np.random.seed(0)
-    A = LinearOperator(Xhat.shape, matvec=f, dtype=np.float64)
+    A = LinearOperator((ns, ns), matvec=f, dtype=np.float64)
     t0 = time.perf_counter()
     eigsh(A, k=min(cfg[2], ns - 2), v0=np.random.rand(ns))
     t_eig_sample = time.perf_counter() - t0
@@ -169,6 +171,7 @@ def cpu_baseline(torch, chunks, cfg, emb, sel_nnz_full):
     # 3) kNN: exact kd-tree (scipy cKDTree, all workers) on the full embedding;
     # the query time of a row sample is scaled to every row
     from scipy.spatial import cKDTree
+    log("[bench] cpu baseline: cKDTree")
     E = emb.cpu().numpy() if hasattr(emb, "cpu") else emb
     t0 = time.perf_counter()
     tree = cKDTree(E)

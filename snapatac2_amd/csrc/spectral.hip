@@ -236,61 +236,106 @@ __global__ void square_f32_kernel(const double* w, int64_t m, float* w2) {
 // ---------------------------------------------------------------------------
 // Operator application (FP32 sparse products)
 // ---------------------------------------------------------------------------
-// Vs[i][c] = r_i * V[i][c] as FP32, columns >= b zero-filled.
-__global__ void scale_in_kernel(const double* V, int64_t ldv, int64_t n, int b, const float* r,
-                                float* Vs) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t i = t >> 5;
-  const int c = t & 31;
-  if (i >= n) return;
-  Vs[i * kB + c] = c < b ? (float)((double)r[i] * V[i * ldv + c]) : 0.f;
-}
-
-// out[row][lane] = post(sum_{e in row} cnt[e] * dense[col[e]][lane]) for 32 lanes
-// per row (two rows per wave). Fixed summation order (entry order).
-template <int EPI, bool FV>
-__global__ void __launch_bounds__(256) spmm32_kernel(const int64_t* __restrict__ ptr,
-                                                     const int32_t* __restrict__ col,
-                                                     const uint32_t* __restrict__ cnt, int64_t nrows,
-                                                     const float* __restrict__ dense,
-                                                     const float* __restrict__ rowscale,
-                                                     float* __restrict__ out32,
-                                                     // EPI 1: FP64 output with the -dinv*V term
-                                                     const double* __restrict__ V, int64_t ldv,
-                                                     const double* __restrict__ dinv, double* __restrict__ Y,
-                                                     int64_t ldy, int b) {
+// One wave per row. Lanes form G = 64/BW groups of BW lanes; group g handles
+// entries g, g+G, g+2G, ... of the row and lane c of the group column c of the
+// dense operand (rows of BW floats: a 4*BW-byte gather per entry). Batches of 64
+// (column, count) pairs are loaded coalesced and broadcast to the groups by
+// ds_bpermute; gathers are issued 8 steps at a time so several are in flight.
+// The G partial sums are combined by a fixed xor tree: the summation order is
+// fixed, so results are bitwise reproducible.
+//   EPI 0: out32[row][c] = rowscale[row] * sum                    (FP32, stride BW)
+//   EPI 1: Y[row][c] = rowscale[row] * sum - dinv[row] * V[row][c] (FP64)
+template <int BW, int EPI, bool FV>
+__global__ void __launch_bounds__(256) spmm_kernel(const int64_t* __restrict__ ptr,
+                                                   const int32_t* __restrict__ col,
+                                                   const uint32_t* __restrict__ cnt, int64_t nrows,
+                                                   const float* __restrict__ dense,
+                                                   const float* __restrict__ rowscale,
+                                                   float* __restrict__ out32,
+                                                   const double* __restrict__ V, int64_t ldv,
+                                                   const double* __restrict__ dinv,
+                                                   double* __restrict__ Y, int64_t ldy, int b) {
+  constexpr int G = 64 / BW;
   const int lane = threadIdx.x & 63;
-  const int half = lane >> 5;
-  const int c = lane & 31;
-  const int64_t row = ((((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) << 1) + half;
-  const bool active = row < nrows;
-  int64_t e0 = 0, e1 = 0;
-  if (active) {
-    e0 = ptr[row];
-    e1 = ptr[row + 1];
-  }
+  const int g = lane / BW;
+  const int c = lane % BW;
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (row >= nrows) return;
+  const int64_t e0 = ptr[row], e1 = ptr[row + 1];
   float acc = 0.f;
-  for (int64_t base = e0; base < e1; base += 32) {
-    const int64_t e = base + c;
-    int32_t j = 0;
-    float v = 0.f;
-    if (e < e1) {
-      j = col[e];
-      v = val32<FV>(cnt, e);
-    }
-    const int nb = (int)min((int64_t)32, e1 - base);
-    for (int t = 0; t < nb; ++t) {
-      const int32_t jt = __shfl(j, t, 32);
-      const float vt = __shfl(v, t, 32);
-      acc = fmaf(vt, dense[(int64_t)jt * kB + c], acc);
-    }
+  int32_t j = 0;
+  float v = 0.f;
+  if (e0 + lane < e1) {
+    j = col[e0 + lane];
+    v = val32<FV>(cnt, e0 + lane);
   }
-  if (!active) return;
+  for (int64_t base = e0; base < e1; base += 64) {
+    int32_t jn = 0;
+    float vn = 0.f;
+    const int64_t en = base + 64 + lane;
+    if (en < e1) {  // prefetch the next batch
+      jn = col[en];
+      vn = val32<FV>(cnt, en);
+    }
+    const int nb = (int)min((int64_t)64, e1 - base);
+    const int steps = (nb + G - 1) / G;  // entries s*G + g, s < steps
+    int s = 0;
+    for (; s + 8 <= steps; s += 8) {
+      float z[8], vv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int t = (s + u) * G + g;
+        const int32_t jt = __shfl(j, t, 64);
+        const float vt = __shfl(v, t, 64);
+        vv[u] = t < nb ? vt : 0.f;
+        z[u] = dense[(int64_t)(t < nb ? jt : 0) * BW + c];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = fmaf(vv[u], z[u], acc);
+    }
+    for (; s < steps; ++s) {
+      const int t = s * G + g;
+      const int32_t jt = __shfl(j, t, 64);
+      const float vt = __shfl(v, t, 64);
+      if (t < nb) acc = fmaf(vt, dense[(int64_t)jt * BW + c], acc);
+    }
+    j = jn;
+    v = vn;
+  }
+#pragma unroll
+  for (int o = BW; o < 64; o <<= 1) acc += __shfl_xor(acc, o, 64);
+  if (g != 0) return;
   if (EPI == 0) {
-    out32[row * kB + c] = acc * rowscale[row];
+    out32[row * BW + c] = acc * rowscale[row];
   } else if (c < b) {
     Y[row * ldy + c] = (double)(acc * rowscale[row]) - dinv[row] * V[row * ldv + c];
   }
+}
+
+// Vs[i][c] = r_i * V[i][c] as FP32 (row stride BW), columns >= b zero-filled.
+template <int BW>
+__global__ void scale_in_kernel(const double* V, int64_t ldv, int64_t n, int b, const float* r,
+                                float* Vs) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t / BW;
+  const int c = (int)(t % BW);
+  if (i >= n) return;
+  Vs[i * BW + c] = c < b ? (float)((double)r[i] * V[i * ldv + c]) : 0.f;
+}
+
+template <int BW, bool FV>
+static void launch_apply(const int64_t* indptr, const int32_t* idx, const uint32_t* cnt, int64_t n,
+                         const int64_t* colptr, const int32_t* rowidx, const uint32_t* ccnt,
+                         int64_t m, const float* r32, const float* w2, const double* dinv,
+                         const double* V, int64_t ldv, int b, double* Y, int64_t ldy, float* Vs,
+                         float* Z, hipStream_t st) {
+  scale_in_kernel<BW><<<(unsigned)ceil_div(n * BW, 256), 256, 0, st>>>(V, ldv, n, b, r32, Vs);
+  // Z = w^2 o (X^T Vs): CSC rows are features
+  spmm_kernel<BW, 0, FV><<<(unsigned)ceil_div(m, 4), 256, 0, st>>>(colptr, rowidx, ccnt, m, Vs, w2, Z,
+                                                                   nullptr, 0, nullptr, nullptr, 0, b);
+  // Y = r o (X Z) - dinv o V
+  spmm_kernel<BW, 1, FV><<<(unsigned)ceil_div(n, 4), 256, 0, st>>>(indptr, idx, cnt, n, Z, r32, nullptr,
+                                                                   V, ldv, dinv, Y, ldy, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -566,21 +611,20 @@ int snapgpu_spectral_apply(const int64_t* d_indptr, const int32_t* d_idx, const 
   hipStream_t st = as_stream(stream);
   float* Vs = (float*)(((uintptr_t)d_ws + 255) & ~(uintptr_t)255);
   float* Z = Vs + n * kB;
-  scale_in_kernel<<<(unsigned)ceil_div(n * 32, 256), 256, 0, st>>>(d_V, ldv, n, b, d_r32, Vs);
-  SG_LAUNCH_CHECK();
-  // Z = w^2 o (X^T Vs)   (CSC rows = features);  Y = r o (X Z) - dinv o V
-  const unsigned gm = (unsigned)ceil_div(m, 8), gn = (unsigned)ceil_div(n, 8);
-  if (value_type == 0) {
-    spmm32_kernel<0, false><<<gm, 256, 0, st>>>(d_colptr, d_rowidx, d_ccnt, m, Vs, d_w2, Z, nullptr, 0,
-                                                nullptr, nullptr, 0, b);
-    spmm32_kernel<1, false><<<gn, 256, 0, st>>>(d_indptr, d_idx, d_cnt, n, Z, d_r32, nullptr, d_V, ldv,
-                                                d_dinv, d_Y, ldy, b);
+  const bool fv = value_type == 1;
+#define SG_APPLY(BW)                                                                       \
+  if (fv) launch_apply<BW, true>(d_indptr, d_idx, d_cnt, n, d_colptr, d_rowidx, d_ccnt, m, d_r32, \
+                                 d_w2, d_dinv, d_V, ldv, b, d_Y, ldy, Vs, Z, st);              \
+  else launch_apply<BW, false>(d_indptr, d_idx, d_cnt, n, d_colptr, d_rowidx, d_ccnt, m, d_r32, \
+                               d_w2, d_dinv, d_V, ldv, b, d_Y, ldy, Vs, Z, st);
+  if (b <= 8) {
+    SG_APPLY(8)
+  } else if (b <= 16) {
+    SG_APPLY(16)
   } else {
-    spmm32_kernel<0, true><<<gm, 256, 0, st>>>(d_colptr, d_rowidx, d_ccnt, m, Vs, d_w2, Z, nullptr, 0,
-                                               nullptr, nullptr, 0, b);
-    spmm32_kernel<1, true><<<gn, 256, 0, st>>>(d_indptr, d_idx, d_cnt, n, Z, d_r32, nullptr, d_V, ldv,
-                                               d_dinv, d_Y, ldy, b);
+    SG_APPLY(32)
   }
+#undef SG_APPLY
   SG_LAUNCH_CHECK();
   return SNAPGPU_OK;
 }

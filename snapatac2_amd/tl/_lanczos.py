@@ -25,16 +25,24 @@ class _Linalg:
         self.L = _lib.lib()
         self.ws_b = int(self.L.snapgpu_gemm_tn_workspace_bytes(n, max_cols, 32))
         self.ws = torch.empty(self.ws_b, dtype=torch.uint8, device=device)
-        self.cbuf = torch.empty(max_cols * 32, dtype=torch.float64, device=device)
+        self.cbufs = [torch.empty(max_cols * 32, dtype=torch.float64, device=device) for _ in range(2)]
         self.bbuf = torch.empty(max_cols * 32, dtype=torch.float64, device=device)
 
-    def tn(self, A, B) -> np.ndarray:
-        """A^T B for device views A (n x ka), B (n x kb <= 32) -> host array."""
+    def tn(self, A, B, slot: int = 0):
+        """A^T B for device views A (n x ka), B (n x kb <= 32) -> device (ka x kb)."""
         ka, kb = A.shape[1], B.shape[1]
+        out = self.cbufs[slot]
         _lib.check(self.L.snapgpu_gemm_tn_f64(A.data_ptr(), A.stride(0), B.data_ptr(), B.stride(0),
-                                              self.n, ka, kb, self.cbuf.data_ptr(), self.ws.data_ptr(),
+                                              self.n, ka, kb, out.data_ptr(), self.ws.data_ptr(),
                                               self.ws_b, _lib.stream_ptr()))
-        return self.cbuf[:ka * kb].cpu().numpy().reshape(ka, kb).copy()
+        return out[:ka * kb].view(ka, kb)
+
+    def nn_dev(self, A, Mdev, Cout, alpha=1.0, beta=0.0) -> None:
+        """Cout = beta*Cout + alpha * A @ Mdev for a contiguous device matrix (ka x kb <= 32)."""
+        ka, kb = Mdev.shape
+        _lib.check(self.L.snapgpu_gemm_nn_f64(A.data_ptr(), A.stride(0), Mdev.data_ptr(), self.n,
+                                              ka, kb, float(alpha), float(beta), Cout.data_ptr(),
+                                              Cout.stride(0), _lib.stream_ptr()))
 
     def nn(self, A, M: np.ndarray, Cout, alpha=1.0, beta=0.0) -> None:
         """Cout = beta*Cout + alpha * A @ M with host matrix M (ka x kb), column chunks of 32."""
@@ -43,14 +51,21 @@ class _Linalg:
             Mc = np.ascontiguousarray(M[:, c0:c0 + 32], dtype=np.float64)
             w = Mc.shape[1]
             self.bbuf[:ka * w].copy_(self.torch.from_numpy(Mc.ravel()))
-            Cv = Cout[:, c0:c0 + w]
-            _lib.check(self.L.snapgpu_gemm_nn_f64(A.data_ptr(), A.stride(0), self.bbuf.data_ptr(),
-                                                  self.n, ka, w, float(alpha), float(beta),
-                                                  Cv.data_ptr(), Cv.stride(0), _lib.stream_ptr()))
+            self.nn_dev(A, self.bbuf[:ka * w].view(ka, w), Cout[:, c0:c0 + w], alpha, beta)
 
     def random(self, out, seed: int) -> None:
         _lib.check(self.L.snapgpu_random_f64(int(seed) & 0xFFFFFFFFFFFFFFFF, out.numel(), out.data_ptr(),
                                              _lib.stream_ptr()))
+
+
+def _eigh_small(H):
+    """Symmetric eigensolve of the projected matrix (single-threaded BLAS: faster at these sizes)."""
+    try:
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(limits=1, user_api="blas"):
+            return np.linalg.eigh(H)
+    except ImportError:  # pragma: no cover
+        return np.linalg.eigh(H)
 
 
 def _lm_order(theta: np.ndarray, k: int) -> np.ndarray:
@@ -79,14 +94,14 @@ def dense_eigsh(op, k: int):
         op.apply(E, A[:, c0:c0 + w], w)
     Ah = A.cpu().numpy()
     Ah = (Ah + Ah.T) / 2
-    theta, S = np.linalg.eigh(Ah)
+    theta, S = _eigh_small(Ah)
     order = _lm_order(theta, k)
     U = torch.from_numpy(np.ascontiguousarray(S[:, order])).to(dev)
     return theta[order], U, {"method": "dense", "applies": op.n_applies}
 
 
-def eigsh_lm(op, k: int, block: int = 32, max_basis: int | None = None, tol: float = 1e-9,
-             seed: int = 0, max_restarts: int = 200, dense_below: int = 1024):
+def eigsh_lm(op, k: int, block: int = 16, max_basis: int | None = None, tol: float = 1e-9,
+             seed: int = 0, max_restarts: int = 400, dense_below: int = 1024, check_every: int = 4):
     """k largest-magnitude eigenpairs of the symmetric operator `op`.
 
     Returns (evals (host), evecs (device tensor, n x k), stats) in LM order.
@@ -98,7 +113,7 @@ def eigsh_lm(op, k: int, block: int = 32, max_basis: int | None = None, tol: flo
         return theta, U, st
     b = block
     if max_basis is None:
-        max_basis = max(10 * b, 3 * k + 2 * b)
+        max_basis = max(20 * b, 3 * k + 2 * b)
     Jmax = int(min(max_basis, (n // b - 1) * b))
     Jmax = max(Jmax, 2 * b)
     dev = op.r64.device
@@ -112,14 +127,14 @@ def eigsh_lm(op, k: int, block: int = 32, max_basis: int | None = None, tol: flo
     def orth_against(Vb, J):
         for _ in range(2):
             C = la.tn(V[:, :J], Vb)
-            la.nn(V[:, :J], C, Vb, alpha=-1.0, beta=1.0)
+            la.nn_dev(V[:, :J], C, Vb, alpha=-1.0, beta=1.0)
 
     def cholqr2(Wsrc, dest):
         """dest = Wsrc R^-1 with R upper (b x b); returns R, or None if rank deficient."""
         Rtot = np.eye(b)
         src = Wsrc
         for it in range(2):
-            G = la.tn(src, src)
+            G = la.tn(src, src).cpu().numpy()
             G = (G + G.T) / 2
             ev = np.linalg.eigvalsh(G)
             if ev[0] <= 1e-26 * max(ev[-1], 1e-300):
@@ -156,24 +171,33 @@ def eigsh_lm(op, k: int, block: int = 32, max_basis: int | None = None, tol: flo
     J = b
     restarts = 0
     steps = 0
+    last_res = np.inf
+    since_check = 0
     while True:
         steps += 1
+        since_check += 1
         op.apply(V[:, J - b:J], W, b)
-        C1 = la.tn(V[:, :J], W)
-        la.nn(V[:, :J], C1, W, alpha=-1.0, beta=1.0)
-        C2 = la.tn(V[:, :J], W)
-        la.nn(V[:, :J], C2, W, alpha=-1.0, beta=1.0)
-        H[:J, J - b:J] = C1 + C2
+        C1 = la.tn(V[:, :J], W, slot=0)
+        la.nn_dev(V[:, :J], C1, W, alpha=-1.0, beta=1.0)
+        C2 = la.tn(V[:, :J], W, slot=1)
+        la.nn_dev(V[:, :J], C2, W, alpha=-1.0, beta=1.0)
+        H[:J, J - b:J] = C1.cpu().numpy() + C2.cpu().numpy()
         R = next_block(J, W)
         H[J:J + b, J - b:J] = R
+        at_restart = J + b > Jmax
+        if not (at_restart or since_check >= check_every or last_res < 100 * tol):
+            J += b
+            continue
+        since_check = 0
         Hs = (H[:J, :J] + H[:J, :J].T) / 2
-        theta, S = np.linalg.eigh(Hs)
+        theta, S = _eigh_small(Hs)
         order = _lm_order(theta, k)
         res = np.linalg.norm(R @ S[J - b:J, order], axis=0)
         scale = max(np.abs(theta).max(), 1e-300)
+        last_res = res.max() / scale
         if np.all(res <= tol * scale):
             break
-        if J + b <= Jmax:
+        if not at_restart:
             J += b
             continue
         if restarts >= max_restarts:
