@@ -284,6 +284,7 @@ def main():
         "cpu_baseline": cpu,
         "spmm_gbps": roof["achieved"] if roof else None,
         "knn_recall_sample": recall,
+        "knn_fallback_rows": info["knn_fallback_rows"],
         "stages_s": {k: round(v, 3) for k, v in timer.t.items()},
         "lanczos": info["lanczos"],
     }

@@ -8,11 +8,13 @@
 // Pipeline (all on the device):
 //  1. prepare: FP64 rows -> augmented 32-wide rows x~ = (x, |x|^2, 0) for the
 //     database and q~ = (x, -1/2, 0) for the queries, so s = x~.q~ = x.q - |x|^2/2
-//     and d^2 = |q|^2 - 2 s. Stored as bf16 hi/lo pairs (x = hi + lo + O(2^-16 x))
-//     and as fp32.
+//     and d^2 = |q|^2 - 2 s. Stored as three bf16 terms (x = hi + mid + lo +
+//     O(2^-24 x)) and as fp32.
 //  2. candidates: a wave owns 16 queries and streams every database point in
-//     tiles of 16. BF16 mode: three v_mfma_f32_16x16x32_bf16 per tile
-//     (hi.hi + lo.hi + hi.lo, fp32 accumulation); FP32 mode: SIMT FMAs. Output
+//     tiles of 16. BF16 mode: six v_mfma_f32_16x16x32_bf16 per tile (every
+//     product of the split terms down to order 2^-24, fp32 accumulation), so the
+//     approximation error is at fp32 level even where d^2 is much smaller than
+//     |q|^2 (spectral embeddings); FP32 mode: SIMT FMAs. Output
 //     lane l holds query l%16 and 4 points of the tile, so the 4 lanes of a query
 //     cover 4 disjoint stripes of the database; each lane keeps its best 32
 //     (sorted in registers) -> 128 candidates per query (>= 2k for k <= 64).
@@ -20,9 +22,14 @@
 //     (sum of (a-b)^2 in dimension order, no fused multiply-add: the reference's
 //     squared_euclidean), ordering by (d^2, index), and checking a certificate:
 //     every non-candidate has approx d^2 >= tau_stripe, so if the k-th exact d^2
-//     is below min(tau) - eps (eps a rigorous bound on |approx - exact|) the set
-//     is exact. Rows that fail are rescanned exactly by the fallback kernel.
+//     is below the smallest exact d^2 a non-candidate can have, the set is exact.
+//     That bound uses |approx - exact| <= c (|x||q| + |x|^2 + |q|^2) together
+//     with d >= | |x| - |q| |, so it scales with the query's own norm rather than
+//     the largest norm in the data. Rows that fail are rescanned exactly by the
+//     fallback kernel.
 //  4. output: k neighbours per row sorted by index, distance = sqrt(d^2).
+#include <cfloat>
+
 #include "common.h"
 
 namespace snapgpu {
@@ -39,12 +46,14 @@ constexpr int kMaxK = 100;
 
 enum Mode : int32_t { kModeFP32 = 0, kModeBF16 = 1 };
 
-// Relative error constants of the approximate d^2 (see the certificate above):
-// |approx - exact| <= c * (M|q| + M^2 + |q|^2), M = max row norm.
-// BF16x3: 3*2^-16 per product term + fp32 accumulation of 96 terms, doubled
-// by d^2 = |q|^2 - 2s, plus rounding of the inputs to fp32: c = 1.5e-4.
-// FP32: 33 * 2^-24 per term, doubled, plus input rounding: c = 1.0e-5.
-constexpr double kEpsBF16 = 1.5e-4;
+// Error constants c of the approximate d^2 (see the certificate above):
+// |approx - exact| <= c * (|x||q| + |x|^2 + |q|^2).
+// BF16 (three-term split, six products): the omitted products and the split
+// remainders are <= ~4.1 * 2^-24 per term, fp32 accumulation of 192 terms adds
+// <= 192 * 2^-24, doubled by d^2 = |q|^2 - 2s, plus fp32 rounding of the inputs
+// and of |q|^2: c = 3e-5.
+// FP32: 32 sequential FMAs (<= 32 * 2^-24), doubled, plus input rounding: c = 1e-5.
+constexpr double kEpsBF16 = 3.0e-5;
 constexpr double kEpsFP32 = 1.0e-5;
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
@@ -54,8 +63,8 @@ __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) prepare_kernel(const double* __restrict__ X, int64_t n,
                                                       int d, int64_t ld, int64_t n_pad,
-                                                      __bf16* xh, __bf16* xl, __bf16* qh,
-                                                      __bf16* ql, float* xf, float* qf,
+                                                      __bf16* xh, __bf16* xm, __bf16* xl, __bf16* qh,
+                                                      __bf16* qm, __bf16* ql, float* xf, float* qf,
                                                       float* qnorm, unsigned int* max_norm_bits) {
   const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
   const int c = threadIdx.x & 31;
@@ -77,11 +86,16 @@ __global__ void __launch_bounds__(256) prepare_kernel(const double* __restrict__
   if (row >= n) qv = 0.0;
   const int64_t o = row * kDim + c;
   const float xv32 = (float)xv, qv32 = (float)qv;
+  // three-term split: each step keeps the next 8 significant bits
   const __bf16 xhv = to_bf16(xv32), qhv = to_bf16(qv32);
+  const float xr1 = xv32 - (float)xhv, qr1 = qv32 - (float)qhv;
+  const __bf16 xmv = to_bf16(xr1), qmv = to_bf16(qr1);
   xh[o] = xhv;
-  xl[o] = to_bf16(xv32 - (float)xhv);
+  xm[o] = xmv;
+  xl[o] = to_bf16(xr1 - (float)xmv);
   qh[o] = qhv;
-  ql[o] = to_bf16(qv32 - (float)qhv);
+  qm[o] = qmv;
+  ql[o] = to_bf16(qr1 - (float)qmv);
   xf[o] = xv32;
   qf[o] = qv32;
   if (c == 0) {
@@ -94,7 +108,7 @@ __global__ void __launch_bounds__(256) prepare_kernel(const double* __restrict__
 // 2. candidates
 // ---------------------------------------------------------------------------
 struct CandParams {
-  const __bf16 *xh, *xl, *qh, *ql;
+  const __bf16 *xh, *xm, *xl, *qh, *qm, *ql;
   const float *xf, *qf, *qnorm;
   int64_t n, n_pad;
   const int32_t* rows;  // optional list of query rows (null = all)
@@ -140,19 +154,27 @@ __global__ void __launch_bounds__(256) cand_kernel(CandParams P) {
 
   if (MODE == kModeBF16) {
     const bf16x8 bq_h = *reinterpret_cast<const bf16x8*>(P.qh + q * kDim + g * 8);
+    const bf16x8 bq_m = *reinterpret_cast<const bf16x8*>(P.qm + q * kDim + g * 8);
     const bf16x8 bq_l = *reinterpret_cast<const bf16x8*>(P.ql + q * kDim + g * 8);
     const bf16x8* xh8 = reinterpret_cast<const bf16x8*>(P.xh);
+    const bf16x8* xm8 = reinterpret_cast<const bf16x8*>(P.xm);
     const bf16x8* xl8 = reinterpret_cast<const bf16x8*>(P.xl);
     bf16x8 ah = xh8[(int64_t)j * 4 + g];
+    bf16x8 am = xm8[(int64_t)j * 4 + g];
     bf16x8 al = xl8[(int64_t)j * 4 + g];
     for (int64_t t = 0; t < ntiles; ++t) {
+      // smallest terms first
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_h, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bq_h, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bq_m, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_l, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bq_h, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_m, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_h, acc, 0, 0, 0);
       if (t + 1 < ntiles) {  // prefetch the next tile's A operands
         const int64_t row = (t + 1) * 16 + j;
         ah = xh8[row * 4 + g];
+        am = xm8[row * 4 + g];
         al = xl8[row * 4 + g];
       }
       const float th = L[kList - 1];
@@ -293,6 +315,31 @@ __device__ void write_row(const FinParams& P, int64_t q, double (&dd)[2], int32_
   wave_sync();
 }
 
+// Smallest exact d^2 a point not kept by a stripe with threshold tau can have.
+// Such a point x has approx d^2 >= tau, so exact d^2 >= tau - c(t r + t^2 + r^2)
+// with t = |x|, r = |q|, and also exact d^2 >= (t - r)^2. Minimising the larger
+// of the two over t in [0, M]: below r both fall, so t = r gives tau - 3c r^2;
+// above r the first falls and the second rises, so the minimum is where they
+// cross (larger root of (1+c)t^2 + (c-2) r t + (1+c) r^2 - tau = 0), capped at M.
+__device__ double nonkept_lower_bound(double tau, double nq, double M, double c) {
+  const double r = sqrt(nq);
+  double best = tau - 3.0 * c * nq;  // t <= r
+  const double a = 1.0 + c, bq = (c - 2.0) * r, cq = (1.0 + c) * nq - tau;
+  const double disc = bq * bq - 4.0 * a * cq;
+  double t = r;
+  if (disc >= 0.0) t = (-bq + sqrt(disc)) / (2.0 * a);
+  if (t < r) t = r;
+  double cross;
+  if (t > M) {
+    cross = tau - c * (M * r + M * M + nq);  // beyond the largest norm: first bound at M
+  } else {
+    cross = (t - r) * (t - r);
+  }
+  best = fmin(best, cross);
+  // guard against rounding in the bound itself
+  return best - 1e-7 * fabs(tau) - 4.0 * DBL_EPSILON * (nq + M * M);
+}
+
 __global__ void __launch_bounds__(256) finalize_kernel(FinParams P) {
   __shared__ double lds_d[4][128];
   __shared__ int32_t lds_i[4][128];
@@ -323,9 +370,7 @@ __global__ void __launch_bounds__(256) finalize_kernel(FinParams P) {
   for (int g = 1; g < 4; ++g) tmin = fminf(tmin, P.tau[qi * 4 + g]);
   const double M = (double)__uint_as_float(*P.max_norm_bits);
   const double nq = (double)P.qnorm[q];
-  const double eps = P.eps_c * (M * sqrt(nq) + M * M + nq) + 1e-300;
-  bool ok = nvalid >= k && (double)tmin - eps > dk;
-  if (nvalid == k && tmin == INFINITY) ok = true;  // every other point is a candidate
+  bool ok = nvalid >= k && (tmin == INFINITY || nonkept_lower_bound(tmin, nq, M, P.eps_c) > dk);
   if (P.force_fallback) ok = false;
   if (!ok) {
     if (lane == 0) {
@@ -412,7 +457,7 @@ __global__ void __launch_bounds__(64) fallback_kernel(FinParams P, const int32_t
 }
 
 struct Workspace {
-  __bf16 *xh, *xl, *qh, *ql;
+  __bf16 *xh, *xm, *xl, *qh, *qm, *ql;
   float *xf, *qf, *qnorm, *tau;
   int32_t *cand, *fail_rows;
   unsigned int *max_norm_bits, *fail_count;
@@ -428,8 +473,10 @@ static int64_t layout(int64_t n, Workspace* w, char* base) {
   };
   Workspace t;
   t.xh = (__bf16*)take(n_pad * kDim * 2);
+  t.xm = (__bf16*)take(n_pad * kDim * 2);
   t.xl = (__bf16*)take(n_pad * kDim * 2);
   t.qh = (__bf16*)take(n_pad * kDim * 2);
+  t.qm = (__bf16*)take(n_pad * kDim * 2);
   t.ql = (__bf16*)take(n_pad * kDim * 2);
   t.xf = (float*)take(n_pad * kDim * 4);
   t.qf = (float*)take(n_pad * kDim * 4);
@@ -478,9 +525,9 @@ int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, 
   SG_HIP(hipMemsetAsync(w.max_norm_bits, 0, 4, st));
   SG_HIP(hipMemsetAsync(w.fail_count, 0, 4, st));
   prepare_kernel<<<(unsigned)ceil_div(n_pad * 32, 256), 256, 0, st>>>(
-      d_X, n, d, ld, n_pad, w.xh, w.xl, w.qh, w.ql, w.xf, w.qf, w.qnorm, w.max_norm_bits);
+      d_X, n, d, ld, n_pad, w.xh, w.xm, w.xl, w.qh, w.qm, w.ql, w.xf, w.qf, w.qnorm, w.max_norm_bits);
   SG_LAUNCH_CHECK();
-  CandParams cp{w.xh, w.xl, w.qh, w.ql, w.xf, w.qf, w.qnorm, n, n_pad, nullptr, n, w.cand, w.tau};
+  CandParams cp{w.xh, w.xm, w.xl, w.qh, w.qm, w.ql, w.xf, w.qf, w.qnorm, n, n_pad, nullptr, n, w.cand, w.tau};
   const unsigned cblocks = (unsigned)ceil_div(ceil_div(n, 16), 4);
   if (mode == kModeBF16) cand_kernel<kModeBF16><<<cblocks, 256, 0, st>>>(cp);
   else cand_kernel<kModeFP32><<<cblocks, 256, 0, st>>>(cp);
