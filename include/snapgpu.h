@@ -158,12 +158,13 @@ int64_t snapgpu_knn_workspace_bytes(int64_t n, int32_t d, int32_t k);
  * with a certificate (failing rows are recomputed exactly), so the neighbour
  * sets equal FP64 brute force with ties broken by index.
  * Outputs: d_nbr[n*k] (int32) and d_dist[n*k] (FP64 Euclidean distance), each
- * row sorted by neighbour index (knn.rs:67). flags bit 0: send every row
- * through the exact fallback (testing). *n_fallback: rows that needed it.
- * Synchronises `stream` once. */
+ * row sorted by neighbour index (knn.rs:67). Rows whose certificate fails are
+ * screened again with exact FP64 distances (*n_fallback counts them) and, if
+ * that certificate fails too, scanned exactly (*n_exact). flags bit 0: send
+ * every row straight to the exact scan (testing). Synchronises `stream`. */
 int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, int32_t mode,
                 int32_t* d_nbr, double* d_dist, void* d_workspace, int64_t workspace_bytes,
-                int32_t flags, int64_t* n_fallback, void* stream);
+                int32_t flags, int64_t* n_fallback, int64_t* n_exact, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Path B: the normalized-cosine operator inside spectral()                  */

@@ -35,7 +35,7 @@ SIGNATURES: dict[str, tuple] = {
                                           _i32, C.c_double, C.c_double, _p, _p, _p, _i64, _p]),
     "snapgpu_knn_workspace_bytes": (_i64, [_i64, _i32, _i32]),
     "snapgpu_knn": (C.c_int, [_p, _i64, _i32, _i64, _i32, _i32, _p, _p, _p, _i64, _i32,
-                              C.POINTER(_i64), _p]),
+                              C.POINTER(_i64), C.POINTER(_i64), _p]),
     "snapgpu_csr_select_count": (C.c_int, [_p, _p, _i32, _i64, _p, _p, _p]),
     "snapgpu_csr_select_fill": (C.c_int, [_p, _p, _i32, _p, _i64, _p, _p, _p, _p, _p]),
     "snapgpu_csr_transpose_workspace_bytes": (_i64, [_i64, _i64]),

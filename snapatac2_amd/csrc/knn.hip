@@ -44,7 +44,7 @@ constexpr int kList = 32;     // candidates kept per lane (stripe)
 constexpr int kCand = 4 * kList;  // candidates per query
 constexpr int kMaxK = 100;
 
-enum Mode : int32_t { kModeFP32 = 0, kModeBF16 = 1 };
+enum Mode : int32_t { kModeFP32 = 0, kModeBF16 = 1, kModeExact = 2 };
 
 // Error constants c of the approximate d^2 (see the certificate above):
 // |approx - exact| <= c * (|x||q| + |x|^2 + |q|^2).
@@ -110,6 +110,9 @@ __global__ void __launch_bounds__(256) prepare_kernel(const double* __restrict__
 struct CandParams {
   const __bf16 *xh, *xm, *xl, *qh, *qm, *ql;
   const float *xf, *qf, *qnorm;
+  const double* X;  // FP64 rows (exact mode)
+  int64_t ld;
+  int d;
   int64_t n, n_pad;
   const int32_t* rows;  // optional list of query rows (null = all)
   int64_t n_rows;
@@ -189,6 +192,32 @@ __global__ void __launch_bounds__(256) cand_kernel(CandParams P) {
         }
       }
     }
+  } else if (MODE == kModeExact) {
+    // second tier: exact FP64 differences; the key is -fl32(d^2)
+    double qv[kMaxData];
+#pragma unroll
+    for (int c = 0; c < kMaxData; ++c) qv[c] = c < P.d ? P.X[q * P.ld + c] : 0.0;
+    for (int64_t t = 0; t < ntiles; ++t) {
+      const float th = L[kList - 1];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t p = t * 16 + g * 4 + v;
+        float s = -INFINITY;
+        if (p < P.n) {
+          const double* xr = P.X + p * P.ld;
+          double acc = 0.0;
+          for (int c = 0; c < P.d; ++c) {
+            const double df = xr[c] - qv[c];
+            acc = fma(df, df, acc);
+          }
+          s = -(float)acc;
+        }
+        const bool ins = s > th;
+        if (__ballot(ins)) {
+          if (ins) list_insert(L, I, s, (int32_t)p);
+        }
+      }
+    }
   } else {
     float qv[kDim];
 #pragma unroll
@@ -220,8 +249,12 @@ __global__ void __launch_bounds__(256) cand_kernel(CandParams P) {
 #pragma unroll
   for (int r = 0; r < kList; ++r) out[r] = I[r];
   // stripe bound on the approximate d^2 of every point this lane did not keep
-  const float nq = P.qnorm[q];
-  P.tau[qi * 4 + g] = (L[kList - 1] == -INFINITY) ? INFINITY : nq - 2.0f * L[kList - 1];
+  if (MODE == kModeExact) {
+    P.tau[qi * 4 + g] = (L[kList - 1] == -INFINITY) ? INFINITY : -L[kList - 1];
+  } else {
+    const float nq = P.qnorm[q];
+    P.tau[qi * 4 + g] = (L[kList - 1] == -INFINITY) ? INFINITY : nq - 2.0f * L[kList - 1];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -284,6 +317,7 @@ struct FinParams {
   const float* qnorm;
   const unsigned int* max_norm_bits;
   double eps_c;
+  int mode;
   int force_fallback;
   int32_t* nbr;     // [n][k]
   double* dist;     // [n][k]
@@ -370,7 +404,15 @@ __global__ void __launch_bounds__(256) finalize_kernel(FinParams P) {
   for (int g = 1; g < 4; ++g) tmin = fminf(tmin, P.tau[qi * 4 + g]);
   const double M = (double)__uint_as_float(*P.max_norm_bits);
   const double nq = (double)P.qnorm[q];
-  bool ok = nvalid >= k && (tmin == INFINITY || nonkept_lower_bound(tmin, nq, M, P.eps_c) > dk);
+  double lb;
+  if (P.mode == kModeExact) {
+    // keys are fl32 of the exact FP64 d^2 (FMA accumulation; within a few ulp of
+    // the rescored value): non-kept points have exact d^2 >= tau (1 - 2^-20)
+    lb = (double)tmin * (1.0 - 0x1p-20) - 64.0 * DBL_MIN;
+  } else {
+    lb = nonkept_lower_bound(tmin, nq, M, P.eps_c);
+  }
+  bool ok = nvalid >= k && (tmin == INFINITY || lb > dk);
   if (P.force_fallback) ok = false;
   if (!ok) {
     if (lane == 0) {
@@ -459,8 +501,8 @@ __global__ void __launch_bounds__(64) fallback_kernel(FinParams P, const int32_t
 struct Workspace {
   __bf16 *xh, *xm, *xl, *qh, *qm, *ql;
   float *xf, *qf, *qnorm, *tau;
-  int32_t *cand, *fail_rows;
-  unsigned int *max_norm_bits, *fail_count;
+  int32_t *cand, *fail_rows, *fail_rows2;
+  unsigned int *max_norm_bits, *fail_count, *fail_count2;
 };
 
 static int64_t layout(int64_t n, Workspace* w, char* base) {
@@ -484,8 +526,10 @@ static int64_t layout(int64_t n, Workspace* w, char* base) {
   t.tau = (float*)take(n * 4 * 4);
   t.cand = (int32_t*)take(n * kCand * 4);
   t.fail_rows = (int32_t*)take(n * 4);
+  t.fail_rows2 = (int32_t*)take(n * 4);
   t.max_norm_bits = (unsigned int*)take(4);
   t.fail_count = (unsigned int*)take(4);
+  t.fail_count2 = (unsigned int*)take(4);
   if (w) *w = t;
   return off;
 }
@@ -507,7 +551,7 @@ int64_t snapgpu_knn_workspace_bytes(int64_t n, int32_t d, int32_t k) {
 
 int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, int32_t mode,
                 int32_t* d_nbr, double* d_dist, void* d_workspace, int64_t workspace_bytes,
-                int32_t flags, int64_t* n_fallback, void* stream) {
+                int32_t flags, int64_t* n_fallback, int64_t* n_exact, void* stream) {
   clear_error();
   SG_ARG(n >= 2, "need at least 2 points");
   SG_ARG(n < INT32_MAX, "too many points");
@@ -527,25 +571,50 @@ int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, 
   prepare_kernel<<<(unsigned)ceil_div(n_pad * 32, 256), 256, 0, st>>>(
       d_X, n, d, ld, n_pad, w.xh, w.xm, w.xl, w.qh, w.qm, w.ql, w.xf, w.qf, w.qnorm, w.max_norm_bits);
   SG_LAUNCH_CHECK();
-  CandParams cp{w.xh, w.xm, w.xl, w.qh, w.qm, w.ql, w.xf, w.qf, w.qnorm, n, n_pad, nullptr, n, w.cand, w.tau};
+  CandParams cp{w.xh, w.xm, w.xl, w.qh, w.qm, w.ql, w.xf, w.qf, w.qnorm, d_X, ld, d,
+                n, n_pad, nullptr, n, w.cand, w.tau};
   const unsigned cblocks = (unsigned)ceil_div(ceil_div(n, 16), 4);
   if (mode == kModeBF16) cand_kernel<kModeBF16><<<cblocks, 256, 0, st>>>(cp);
   else cand_kernel<kModeFP32><<<cblocks, 256, 0, st>>>(cp);
   SG_LAUNCH_CHECK();
   FinParams fp{d_X, n, ld, d, k, nullptr, n, w.cand, w.tau, w.qnorm, w.max_norm_bits,
-               mode == kModeBF16 ? kEpsBF16 : kEpsFP32, (flags & 1) ? 1 : 0,
+               mode == kModeBF16 ? kEpsBF16 : kEpsFP32, mode, (flags & 1) ? 1 : 0,
                d_nbr, d_dist, w.fail_rows, w.fail_count};
   finalize_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, st>>>(fp);
   SG_LAUNCH_CHECK();
-  unsigned int nfail = 0;
+  unsigned int nfail = 0, nfail2 = 0;
   SG_HIP(hipMemcpyAsync(&nfail, w.fail_count, 4, hipMemcpyDeviceToHost, st));
   SG_HIP(hipStreamSynchronize(st));
-  if (nfail > 0) {
-    const unsigned blocks = (unsigned)std::min<int64_t>(nfail, 65535);
-    fallback_kernel<<<blocks, 64, 0, st>>>(fp, w.fail_rows, w.fail_count);
+  if (nfail > 0 && !(flags & 1)) {
+    // second tier: exact FP64 distances for the failed rows, then the certificate again
+    SG_HIP(hipMemsetAsync(w.fail_count2, 0, 4, st));
+    CandParams cp2 = cp;
+    cp2.rows = w.fail_rows;
+    cp2.n_rows = nfail;
+    cand_kernel<kModeExact><<<(unsigned)ceil_div(ceil_div((int64_t)nfail, 16), 4), 256, 0, st>>>(cp2);
+    SG_LAUNCH_CHECK();
+    FinParams fp2 = fp;
+    fp2.rows = w.fail_rows;
+    fp2.n_rows = nfail;
+    fp2.mode = kModeExact;
+    fp2.fail_rows = w.fail_rows2;
+    fp2.fail_count = w.fail_count2;
+    finalize_kernel<<<(unsigned)ceil_div((int64_t)nfail, 4), 256, 0, st>>>(fp2);
+    SG_LAUNCH_CHECK();
+    SG_HIP(hipMemcpyAsync(&nfail2, w.fail_count2, 4, hipMemcpyDeviceToHost, st));
+    SG_HIP(hipStreamSynchronize(st));
+    if (nfail2 > 0) {
+      fallback_kernel<<<(unsigned)std::min<int64_t>(nfail2, 65535), 64, 0, st>>>(fp, w.fail_rows2,
+                                                                                 w.fail_count2);
+      SG_LAUNCH_CHECK();
+    }
+  } else if (nfail > 0) {
+    fallback_kernel<<<(unsigned)std::min<int64_t>(nfail, 65535), 64, 0, st>>>(fp, w.fail_rows,
+                                                                              w.fail_count);
     SG_LAUNCH_CHECK();
   }
-  if (n_fallback) *n_fallback = nfail;
+  if (n_fallback) *n_fallback = (int64_t)nfail;
+  if (n_exact) *n_exact = (int64_t)(flags & 1 ? nfail : nfail2);
   return SNAPGPU_OK;
 }
 

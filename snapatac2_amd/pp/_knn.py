@@ -37,18 +37,20 @@ def nearest_neighbour_graph(data, k: int, precision: str = "bf16", force_fallbac
     if kk < 1:
         # the kd-tree returns no neighbours for a single point
         out = csr_matrix((n, n), dtype=np.float64)
-        return (out, {"fallback_rows": 0}) if return_stats else out
+        return (out, {"fallback_rows": 0, "exact_scan_rows": 0}) if return_stats else out
     nbr = torch.empty((n, kk), dtype=torch.int32, device=X.device)
     dist = torch.empty((n, kk), dtype=torch.float64, device=X.device)
     ws_b = int(L.snapgpu_knn_workspace_bytes(n, d, kk))
     ws = torch.empty(ws_b, dtype=torch.uint8, device=X.device)
     nfb = C.c_int64(0)
+    nex = C.c_int64(0)
     _lib.check(L.snapgpu_knn(X.data_ptr(), n, d, d, kk, _MODES[precision], nbr.data_ptr(),
                              dist.data_ptr(), ws.data_ptr(), ws_b, 1 if force_fallback else 0,
-                             C.byref(nfb), _lib.stream_ptr()))
+                             C.byref(nfb), C.byref(nex), _lib.stream_ptr()))
     indptr = np.arange(0, n * kk + 1, kk, dtype=np.int64)
     out = csr_matrix((dist.cpu().numpy().ravel(), nbr.cpu().numpy().ravel(), indptr), shape=(n, n))
-    return (out, {"fallback_rows": int(nfb.value)}) if return_stats else out
+    stats = {"fallback_rows": int(nfb.value), "exact_scan_rows": int(nex.value)}
+    return (out, stats) if return_stats else out
 
 
 def knn(
