@@ -210,6 +210,8 @@ def main():
     ap.add_argument("--workload", default="pipeline-1m", choices=sorted(WORKLOADS))
     ap.add_argument("--cells", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run one copy kernel of MIB MiB (calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X; "
@@ -218,6 +220,15 @@ def main():
     import torch
     from snapatac2_amd import synth
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        from snapatac2_amd import _lib
+        nb = args.copy_calibration << 20
+        src = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        _lib.check(_lib.lib().snapgpu_copy_bytes(src.data_ptr(), dst.data_ptr(), nb, _lib.stream_ptr()))
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: {nb} bytes read, {nb} bytes written")
+        del src, dst
     cfg = list(WORKLOADS[args.workload])
     if args.cells:
         cfg[0] = args.cells

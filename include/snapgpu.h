@@ -42,6 +42,11 @@ enum snapgpu_strategy {
 const char* snapgpu_last_error(void);
 int snapgpu_abi_version(void);
 
+/* Streaming device-to-device copy of `bytes` (multiple of 16, 16-byte aligned
+ * pointers). A kernel of known traffic, used to calibrate the hardware byte
+ * counters that profiles report. */
+int snapgpu_copy_bytes(const void* d_src, void* d_dst, int64_t bytes, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Prefix sums                                                              */
 /* ------------------------------------------------------------------------ */

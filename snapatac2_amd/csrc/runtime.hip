@@ -70,6 +70,13 @@ __global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(int64_t* data,
   }
 }
 
+// Plain streaming copy (16-byte vectors); used to calibrate hardware byte counters.
+__global__ void __launch_bounds__(256) copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                   int64_t n16) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 static int64_t scan_ws_elems(int64_t n) {
   int64_t total = 0;
   while (n > kScanTile) {
@@ -105,6 +112,18 @@ extern "C" {
 const char* snapgpu_last_error(void) { return g_last_error.c_str(); }
 
 int snapgpu_abi_version(void) { return 1; }
+
+int snapgpu_copy_bytes(const void* d_src, void* d_dst, int64_t bytes, void* stream) {
+  clear_error();
+  SG_ARG(bytes >= 0 && bytes % 16 == 0, "bytes must be a multiple of 16");
+  SG_ARG(((uintptr_t)d_src % 16) == 0 && ((uintptr_t)d_dst % 16) == 0, "pointers must be 16-byte aligned");
+  if (bytes == 0) return SNAPGPU_OK;
+  const int64_t n16 = bytes / 16;
+  copy_kernel<<<(unsigned)std::min<int64_t>(ceil_div(n16, 256), 8192), 256, 0, as_stream(stream)>>>(
+      static_cast<const uint4*>(d_src), static_cast<uint4*>(d_dst), n16);
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
 
 int64_t snapgpu_scan_workspace_bytes(int64_t n) { return scan_ws_elems(n) * 8 + 8; }
 
