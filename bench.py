@@ -53,7 +53,9 @@ class StageTimer:
 
             def __exit__(self_, *exc):
                 timer.torch.cuda.synchronize()
-                timer.t[name] = timer.t.get(name, 0.0) + time.perf_counter() - self_.t0
+                dt = time.perf_counter() - self_.t0
+                timer.t[name] = timer.t.get(name, 0.0) + dt
+                log(f"[bench]   {name}: {dt:.3f}s")
         return _Ctx()
 
 
