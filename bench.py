@@ -103,23 +103,37 @@ def run_pipeline(torch, chunks, cfg, timer, spmm_events):
     return info, emb, g
 
 
-def spmm_roofline(torch, op, events, k_cols=32):
-    """Algorithmic bytes of one operator application over its measured average
-    duration (HIP events on the launching stream)."""
+def spmm_algorithmic_bytes(n, m, nnz, b):
+    """Algorithmic bytes of one operator application on a block of b columns
+    (b-wide FP32 dense rows; int32 index + uint32 count per stored entry; int64
+    row pointers). Every array is counted once:
+      CSC pass: colptr, row ids + counts, gathered Vs (n x b), Z written (m x b), w^2
+      CSR pass: indptr, column ids + counts, gathered Z (m x b), r, dinv, V read, Y written
+      scale:    V read, r, Vs written."""
+    f32, f64 = 4, 8
+    csc = (m + 1) * 8 + nnz * 8 + n * b * f32 + m * b * f32 + m * f32
+    csr = (n + 1) * 8 + nnz * 8 + m * b * f32 + n * f32 + n * f64 + n * b * f64 + n * b * f64
+    scale = n * b * f64 + n * f32 + n * b * f32
+    return csc + csr + scale
+
+
+def spmm_roofline(torch, op, events, block, workload):
+    """Achieved GB/s of one operator application: algorithmic bytes over the
+    average duration measured with HIP events on the launching stream."""
     if not events:
         return None
     durs = [a.elapsed_time(b) for a, b in events]
     avg_ms = float(np.mean(durs))
-    n, m, nnz = op.n, op.m, op.X.nnz
-    # per pass: int32 column/row index + uint32 count per entry, int64 row pointers;
-    # dense: V (FP64) read, Z (FP32, 32 wide) written and read, Y (FP64) written,
-    # plus r, dinv, w^2 vectors.
-    alg = 2 * (nnz * 8 + (n + 1) * 8) + n * k_cols * 8 + 2 * m * 32 * 4 + n * k_cols * 8 \
-        + n * k_cols * 8 + 8 * n + 4 * m + 4 * n
+    alg = spmm_algorithmic_bytes(op.n, op.m, op.X.nnz, block)
     gbs = alg / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", f"spmm_pair_{workload}_traffic.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            traffic = int(json.load(f)["traffic_bytes_per_application"])
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "spectral operator application (scale + CSC SpMM + CSR SpMM, 32 columns)",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": f"spectral operator application (scale + CSC SpMM + CSR SpMM, {block} columns)",
             "avg_ms": round(avg_ms, 3), "launches": len(durs), "alg_bytes": int(alg)}
 
 
@@ -261,7 +275,8 @@ def main():
             + json.dumps({k: round(v, 3) for k, v in timer.t.items()}))
         if timed:
             times.append(dt)
-            roof = spmm_roofline(torch, info["op"], list(spmm_events))
+            roof = spmm_roofline(torch, info["op"], list(spmm_events), info["lanczos"]["block"],
+                                 args.workload)
     ms = float(np.mean(times)) * 1e3
     value = n_cells / (ms / 1e3)
     log(f"[bench] lanczos {info['lanczos']}  nnz tile={info['tile_nnz']} sel={info['selected_nnz']}")
