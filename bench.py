@@ -275,7 +275,7 @@ def main():
             + json.dumps({k: round(v, 3) for k, v in timer.t.items()}))
         if timed:
             times.append(dt)
-            roof = spmm_roofline(torch, info["op"], list(spmm_events), info["lanczos"]["block"],
+            roof = spmm_roofline(torch, info["op"], list(spmm_events), info["lanczos"].get("block", 16),
                                  args.workload)
     ms = float(np.mean(times)) * 1e3
     value = n_cells / (ms / 1e3)
