@@ -30,6 +30,8 @@
 //  4. output: k neighbours per row sorted by index, distance = sqrt(d^2).
 #include <cfloat>
 
+#include <hipcub/hipcub.hpp>
+
 #include "common.h"
 
 namespace snapgpu {
@@ -133,6 +135,206 @@ __device__ __forceinline__ void list_insert(float (&L)[kList], int32_t (&I)[kLis
     L[0] = s;
     I[0] = p;
   }
+}
+
+// ---------------------------------------------------------------------------
+// 2b. spatial query order + per-workgroup centres (BF16 path)
+// ---------------------------------------------------------------------------
+// The BF16 error bound scales with the squared norms of the (shifted) vectors.
+// Queries are sorted along a Z-order curve of their leading coordinates so that
+// the 128 queries of a workgroup are spatial neighbours; every vector is then
+// shifted by the workgroup's centre, which makes the norms (and the bound)
+// comparable to within-group distances instead of distances from the origin.
+constexpr int kKeyDims = 9;   // dimensions in the Z-order key
+constexpr int kKeyBits = 7;   // bits per dimension (63-bit key)
+constexpr int kWgWaves = 8;   // waves per workgroup in the centred kernel
+constexpr int kWgQueries = kWgWaves * 16;
+constexpr int kChunk = 128;   // database points staged in LDS per step
+
+__device__ __forceinline__ unsigned int f2ord(float f) {
+  const unsigned int u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(unsigned int o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
+__global__ void mm_init_kernel(unsigned int* mm) {
+  const int t = threadIdx.x;
+  if (t < 2 * kKeyDims) mm[t] = (t & 1) ? 0u : 0xFFFFFFFFu;
+}
+
+__global__ void minmax_kernel(const double* X, int64_t n, int d, int64_t ld, unsigned int* mm) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int kd = d < kKeyDims ? d : kKeyDims;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * kd; t += stride) {
+    const int64_t i = t / kd;
+    const int c = (int)(t % kd);
+    const unsigned int o = f2ord((float)X[i * ld + c]);
+    atomicMin(&mm[2 * c], o);
+    atomicMax(&mm[2 * c + 1], o);
+  }
+}
+
+__global__ void zkey_kernel(const double* X, int64_t n, int d, int64_t ld, const unsigned int* mm,
+                            uint64_t* keys, uint32_t* idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int kd = d < kKeyDims ? d : kKeyDims;
+  uint64_t key = 0;
+  for (int b = kKeyBits - 1; b >= 0; --b) {
+    for (int c = 0; c < kd; ++c) {
+      const float lo = ord2f(mm[2 * c]), hi = ord2f(mm[2 * c + 1]);
+      const float span = hi - lo;
+      float u = span > 0.f ? ((float)X[i * ld + c] - lo) / span : 0.f;
+      u = fminf(fmaxf(u, 0.f), 1.f);
+      const uint32_t q = (uint32_t)fminf(u * (float)(1 << kKeyBits), (float)((1 << kKeyBits) - 1));
+      key = (key << 1) | ((q >> b) & 1u);
+    }
+  }
+  keys[i] = key;
+  idx[i] = (uint32_t)i;
+}
+
+// centre[w][c] = mean over the workgroup's queries of X[row][c] (FP64 sum, fixed order)
+__global__ void centre_kernel(const double* X, int64_t n, int d, int64_t ld, const uint32_t* order,
+                              float* centre) {
+  const int64_t w = blockIdx.x;
+  const int c = threadIdx.x;  // 32 threads
+  double s = 0.0;
+  int cnt = 0;
+  for (int q = 0; q < kWgQueries; ++q) {
+    const int64_t qi = w * kWgQueries + q;
+    if (qi >= n) break;
+    if (c < d) s += X[(int64_t)order[qi] * ld + c];
+    ++cnt;
+  }
+  centre[w * kDim + c] = (c < d && cnt > 0) ? (float)(s / cnt) : 0.f;
+}
+
+struct CentredParams {
+  const double* X;       // FP64 rows (leading dimension ld)
+  int64_t ld;
+  const float* centre;   // [n_wg][32]
+  const uint32_t* order; // sorted position -> row
+  int64_t n, n_pad;
+  int d;
+  int32_t* cand;         // [n][kCand] by sorted position
+  float* tau;            // [n][4]
+  float* qnorm_c;        // [n] |q - centre|^2 by sorted position
+};
+
+__device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
+  h = to_bf16(x);
+  const float r1 = x - (float)h;
+  m = to_bf16(r1);
+  l = to_bf16(r1 - (float)m);
+}
+
+// Workgroup = 8 waves = 128 consecutive sorted queries sharing one centre.
+// Database chunks of 128 points are loaded once per workgroup, shifted by the
+// centre, given their augmented coordinate |x - c|^2, split into three bf16
+// terms and staged in LDS; every wave then runs its 16 x 16 MFMA tiles on them.
+__global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredParams P) {
+  __shared__ __bf16 As[3][kChunk][kDim];
+  __shared__ float cen[kDim];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  const int64_t wg = blockIdx.x;
+  if (tid < kDim) cen[tid] = P.centre[wg * kDim + tid];
+  __syncthreads();
+  const int64_t qi = wg * kWgQueries + wv * 16 + j;  // sorted position
+  const bool qvalid = qi < P.n;
+  const int64_t q = qvalid ? (int64_t)P.order[qi] : 0;
+  // query operand: (q - c, -1/2 at dim 30), split in three bf16 terms
+  bf16x8 bq_h, bq_m, bq_l;
+  float part = 0.f;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int c = g * 8 + e;
+    float v = 0.f;
+    // centre in FP64, then round: the rounding error is relative to the shifted value
+    if (c < P.d) v = (float)(P.X[q * P.ld + c] - (double)cen[c]);
+    if (c == kMaxData) v = -0.5f;
+    if (c < P.d) part = fmaf(v, v, part);
+    __bf16 h, m, l;
+    split3(v, h, m, l);
+    bq_h[e] = h;
+    bq_m[e] = m;
+    bq_l[e] = l;
+  }
+  part += __shfl_xor(part, 16, 64);
+  part += __shfl_xor(part, 32, 64);
+  const float nqc = part;  // |q - c|^2 (fp32)
+
+  float L[kList];
+  int32_t I[kList];
+#pragma unroll
+  for (int r = 0; r < kList; ++r) {
+    L[r] = -INFINITY;
+    I[r] = -1;
+  }
+  // staging roles: thread t -> point t/4, dims (t%4)*8 .. +7
+  const int sp = tid >> 2, sq = tid & 3;
+  for (int64_t c0 = 0; c0 < P.n_pad; c0 += kChunk) {
+    __syncthreads();
+    for (int pp = sp; pp < kChunk; pp += (kWgWaves * 64) / 4) {
+      const int64_t p = c0 + pp;
+      float v[8];
+      float ps = 0.f;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int c = sq * 8 + e;
+        v[e] = 0.f;
+        if (p < P.n && c < P.d) v[e] = (float)(P.X[p * P.ld + c] - (double)cen[c]);
+        ps = fmaf(v[e], v[e], ps);
+      }
+      ps += __shfl_xor(ps, 1, 64);
+      ps += __shfl_xor(ps, 2, 64);
+      if (sq == 3) v[6] = ps;  // dim 30 = |x - c|^2
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        __bf16 h, m, l;
+        split3(v[e], h, m, l);
+        As[0][pp][sq * 8 + e] = h;
+        As[1][pp][sq * 8 + e] = m;
+        As[2][pp][sq * 8 + e] = l;
+      }
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int tt = 0; tt < kChunk / 16; ++tt) {
+      const int row = tt * 16 + j;
+      const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&As[0][row][g * 8]);
+      const bf16x8 am = *reinterpret_cast<const bf16x8*>(&As[1][row][g * 8]);
+      const bf16x8 al = *reinterpret_cast<const bf16x8*>(&As[2][row][g * 8]);
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bq_h, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bq_m, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_l, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bq_h, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_m, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_h, acc, 0, 0, 0);
+      const float th = L[kList - 1];
+      const int64_t pbase = c0 + tt * 16 + g * 4;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float sv = acc[v];
+        const int64_t p = pbase + v;
+        const bool ins = sv > th && p < P.n;
+        if (__ballot(ins)) {
+          if (ins) list_insert(L, I, sv, (int32_t)p);
+        }
+      }
+    }
+  }
+  if (!qvalid) return;
+  int32_t* out = P.cand + qi * kCand + g * kList;
+#pragma unroll
+  for (int r = 0; r < kList; ++r) out[r] = I[r];
+  P.tau[qi * 4 + g] = (L[kList - 1] == -INFINITY) ? INFINITY : nqc - 2.0f * L[kList - 1];
+  if (g == 0) P.qnorm_c[qi] = nqc;
 }
 
 template <int MODE>
@@ -315,6 +517,7 @@ struct FinParams {
   const int32_t* cand;
   const float* tau;
   const float* qnorm;
+  const float* qnorm_qi;  // if set: |q - centre|^2 by list position (centred BF16 path)
   const unsigned int* max_norm_bits;
   double eps_c;
   int mode;
@@ -371,7 +574,8 @@ __device__ double nonkept_lower_bound(double tau, double nq, double M, double c)
   }
   best = fmin(best, cross);
   // guard against rounding in the bound itself
-  return best - 1e-7 * fabs(tau) - 4.0 * DBL_EPSILON * (nq + M * M);
+  const double mm = M < INFINITY ? M * M : 0.0;  // no cap when the norms are unbounded
+  return best - 1e-7 * fabs(tau) - 4.0 * DBL_EPSILON * (nq + mm);
 }
 
 __global__ void __launch_bounds__(256) finalize_kernel(FinParams P) {
@@ -403,14 +607,14 @@ __global__ void __launch_bounds__(256) finalize_kernel(FinParams P) {
   float tmin = P.tau[qi * 4 + 0];
   for (int g = 1; g < 4; ++g) tmin = fminf(tmin, P.tau[qi * 4 + g]);
   const double M = (double)__uint_as_float(*P.max_norm_bits);
-  const double nq = (double)P.qnorm[q];
+  const double nq = P.qnorm_qi ? (double)P.qnorm_qi[qi] : (double)P.qnorm[q];
   double lb;
   if (P.mode == kModeExact) {
     // keys are fl32 of the exact FP64 d^2 (FMA accumulation; within a few ulp of
     // the rescored value): non-kept points have exact d^2 >= tau (1 - 2^-20)
     lb = (double)tmin * (1.0 - 0x1p-20) - 64.0 * DBL_MIN;
   } else {
-    lb = nonkept_lower_bound(tmin, nq, M, P.eps_c);
+    lb = nonkept_lower_bound(tmin, nq, P.qnorm_qi ? INFINITY : M, P.eps_c);
   }
   bool ok = nvalid >= k && (tmin == INFINITY || lb > dk);
   if (P.force_fallback) ok = false;
@@ -498,11 +702,169 @@ __global__ void __launch_bounds__(64) fallback_kernel(FinParams P, const int32_t
   }
 }
 
+// ---------------------------------------------------------------------------
+// Second tier for the rows whose certificate failed: exact FP64 distances, with
+// the database split into S segments (grid.y) so that a handful of rows still
+// fills the GPU. Each (16 queries, segment) wave keeps 4 x 32 candidates; the
+// keys are -fl32(d^2).
+// ---------------------------------------------------------------------------
+struct SegParams {
+  const double* X;
+  int64_t ld, n;
+  int d;
+  const int32_t* rows;
+  int64_t n_rows;
+  int64_t seg_size;  // multiple of 16
+  int S;
+  int32_t* cand;  // [n_rows][S][kCand]
+  float* tau;     // [n_rows][S][4]  (d^2 bound per stripe)
+};
+
+__global__ void __launch_bounds__(256) exact_seg_kernel(SegParams P) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int seg = blockIdx.y;
+  const int j = lane & 15, g = lane >> 4;
+  const int64_t qi = wave * 16 + j;
+  if (wave * 16 >= P.n_rows) return;
+  const bool qvalid = qi < P.n_rows;
+  const int64_t q = qvalid ? (int64_t)P.rows[qi] : 0;
+  double qv[kMaxData];
+#pragma unroll
+  for (int c = 0; c < kMaxData; ++c) qv[c] = c < P.d ? P.X[q * P.ld + c] : 0.0;
+  float L[kList];
+  int32_t I[kList];
+#pragma unroll
+  for (int r = 0; r < kList; ++r) {
+    L[r] = -INFINITY;
+    I[r] = -1;
+  }
+  const int64_t p0 = (int64_t)seg * P.seg_size;
+  const int64_t p1 = min(P.n, p0 + P.seg_size);
+  for (int64_t t0 = p0; t0 < p1; t0 += 16) {
+    const float th = L[kList - 1];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t p = t0 + g * 4 + v;
+      float sv = -INFINITY;
+      if (p < p1) {
+        const double* xr = P.X + p * P.ld;
+        double acc = 0.0;
+        for (int c = 0; c < P.d; ++c) {
+          const double df = xr[c] - qv[c];
+          acc = fma(df, df, acc);
+        }
+        sv = -(float)acc;
+      }
+      const bool ins = sv > th;
+      if (__ballot(ins)) {
+        if (ins) list_insert(L, I, sv, (int32_t)p);
+      }
+    }
+  }
+  if (!qvalid) return;
+  const int64_t slot = qi * P.S + seg;
+  int32_t* out = P.cand + slot * kCand + g * kList;
+#pragma unroll
+  for (int r = 0; r < kList; ++r) out[r] = I[r];
+  P.tau[slot * 4 + g] = (L[kList - 1] == -INFINITY) ? INFINITY : -L[kList - 1];
+}
+
+// One 256-thread block per row: exact rescoring of the S x 128 candidates,
+// bitonic sort by (d^2, index) in LDS, certificate, output or hand-off.
+constexpr int kMaxSeg = 32;
+__global__ void __launch_bounds__(256) finalize_multi_kernel(FinParams P, SegParams Q,
+                                                             int32_t* fail_rows, unsigned int* fail_count) {
+  __shared__ double sd[kMaxSeg * kCand];
+  __shared__ int32_t si[kMaxSeg * kCand];
+  __shared__ float stau[256];
+  const int64_t qi = blockIdx.x;
+  if (qi >= Q.n_rows) return;
+  const int64_t q = Q.rows[qi];
+  const int C = Q.S * kCand;
+  int Cp = 1;
+  while (Cp < C) Cp <<= 1;
+  const double* xq = P.X + q * P.ld;
+  for (int e = threadIdx.x; e < Cp; e += blockDim.x) {
+    double dv = INFINITY;
+    int32_t iv = INT32_MAX;
+    if (e < C) {
+      const int32_t c = Q.cand[qi * (int64_t)C + e];
+      if (c >= 0 && c != q) {
+        dv = exact_d2(xq, P.X + (int64_t)c * P.ld, P.d);
+        iv = c;
+      }
+    }
+    sd[e] = dv;
+    si[e] = iv;
+  }
+  float tm = INFINITY;
+  for (int e = threadIdx.x; e < Q.S * 4; e += blockDim.x) tm = fminf(tm, Q.tau[qi * (int64_t)Q.S * 4 + e]);
+  stau[threadIdx.x] = tm;
+  __syncthreads();
+  for (int size = 2; size <= Cp; size <<= 1) {
+    for (int jj = size >> 1; jj > 0; jj >>= 1) {
+      for (int e = threadIdx.x; e < Cp; e += blockDim.x) {
+        const int o = e ^ jj;
+        if (o > e) {
+          const bool asc = (e & size) == 0;
+          const bool gt = key_less(sd[o], si[o], sd[e], si[e]);  // element e larger than o
+          if (gt == asc) {
+            const double td = sd[e];
+            const int32_t ti = si[e];
+            sd[e] = sd[o];
+            si[e] = si[o];
+            sd[o] = td;
+            si[o] = ti;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) {
+    float t = INFINITY;
+    for (int e = 0; e < (int)blockDim.x; ++e) t = fminf(t, stau[e]);
+    stau[0] = t;
+  }
+  __syncthreads();
+  const int k = P.k;
+  const float tmin = stau[0];
+  const bool enough = si[k - 1] != INT32_MAX;
+  const double lb = (double)tmin * (1.0 - 0x1p-20) - 64.0 * DBL_MIN;
+  const bool ok = enough && (tmin == INFINITY || lb > sd[k - 1]);
+  if (!ok) {
+    if (threadIdx.x == 0) {
+      const unsigned int slot = atomicAdd(fail_count, 1u);
+      fail_rows[slot] = (int32_t)q;
+    }
+    return;
+  }
+  for (int e = threadIdx.x; e < k; e += blockDim.x) {
+    const int32_t me = si[e];
+    int rank = 0;
+    for (int o = 0; o < k; ++o) rank += si[o] < me;
+    P.nbr[q * k + rank] = me;
+    P.dist[q * k + rank] = sqrt(sd[e]);
+  }
+}
+
+static inline unsigned grid_knn(int64_t work) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, 256), 16384));
+}
+
 struct Workspace {
   __bf16 *xh, *xm, *xl, *qh, *qm, *ql;
   float *xf, *qf, *qnorm, *tau;
   int32_t *cand, *fail_rows, *fail_rows2;
   unsigned int *max_norm_bits, *fail_count, *fail_count2;
+  float *centre, *qnorm_c, *tau2;
+  int32_t* cand2;
+  uint64_t *keys, *keys2;
+  uint32_t *order0, *order;
+  unsigned int* mm;
+  void* sort_tmp;
+  size_t sort_bytes;
 };
 
 static int64_t layout(int64_t n, Workspace* w, char* base) {
@@ -530,6 +892,22 @@ static int64_t layout(int64_t n, Workspace* w, char* base) {
   t.max_norm_bits = (unsigned int*)take(4);
   t.fail_count = (unsigned int*)take(4);
   t.fail_count2 = (unsigned int*)take(4);
+  t.centre = (float*)take(ceil_div(n, kWgQueries) * kDim * 4);
+  const int64_t t2rows = std::max<int64_t>(n, 2048);  // rows x segments of the second tier
+  t.cand2 = (int32_t*)take(t2rows * kCand * 4);
+  t.tau2 = (float*)take(t2rows * 4 * 4);
+  t.qnorm_c = (float*)take(n * 4);
+  t.keys = (uint64_t*)take(n * 8);
+  t.keys2 = (uint64_t*)take(n * 8);
+  t.order0 = (uint32_t*)take(n * 4);
+  t.order = (uint32_t*)take(n * 4);
+  t.mm = (unsigned int*)take(2 * kKeyDims * 4);
+  size_t sb = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sb, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                           (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (int)std::min<int64_t>(n, INT32_MAX));
+  t.sort_bytes = sb;
+  t.sort_tmp = take((int64_t)sb + 256);
   if (w) *w = t;
   return off;
 }
@@ -573,33 +951,47 @@ int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, 
   SG_LAUNCH_CHECK();
   CandParams cp{w.xh, w.xm, w.xl, w.qh, w.qm, w.ql, w.xf, w.qf, w.qnorm, d_X, ld, d,
                 n, n_pad, nullptr, n, w.cand, w.tau};
-  const unsigned cblocks = (unsigned)ceil_div(ceil_div(n, 16), 4);
-  if (mode == kModeBF16) cand_kernel<kModeBF16><<<cblocks, 256, 0, st>>>(cp);
-  else cand_kernel<kModeFP32><<<cblocks, 256, 0, st>>>(cp);
-  SG_LAUNCH_CHECK();
-  FinParams fp{d_X, n, ld, d, k, nullptr, n, w.cand, w.tau, w.qnorm, w.max_norm_bits,
+  FinParams fp{d_X, n, ld, d, k, nullptr, n, w.cand, w.tau, w.qnorm, nullptr, w.max_norm_bits,
                mode == kModeBF16 ? kEpsBF16 : kEpsFP32, mode, (flags & 1) ? 1 : 0,
                d_nbr, d_dist, w.fail_rows, w.fail_count};
+  if (mode == kModeBF16) {
+    // spatial order of the queries, per-workgroup centres, centred LDS-staged MFMA tiles
+    mm_init_kernel<<<1, 64, 0, st>>>(w.mm);
+    minmax_kernel<<<grid_knn(n * std::min(d, kKeyDims)), 256, 0, st>>>(d_X, n, d, ld, w.mm);
+    zkey_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(d_X, n, d, ld, w.mm, w.keys, w.order0);
+    SG_LAUNCH_CHECK();
+    size_t sb = w.sort_bytes;
+    SG_HIP(hipcub::DeviceRadixSort::SortPairs(w.sort_tmp, sb, w.keys, w.keys2, w.order0, w.order,
+                                              (int)n, 0, kKeyDims * kKeyBits, st));
+    const int64_t n_wg = ceil_div(n, kWgQueries);
+    centre_kernel<<<(unsigned)n_wg, 32, 0, st>>>(d_X, n, d, ld, w.order, w.centre);
+    CentredParams cc{d_X, ld, w.centre, w.order, n, n_pad, d, w.cand, w.tau, w.qnorm_c};
+    cand_centred_kernel<<<(unsigned)n_wg, kWgWaves * 64, 0, st>>>(cc);
+    SG_LAUNCH_CHECK();
+    fp.rows = reinterpret_cast<const int32_t*>(w.order);
+    fp.qnorm_qi = w.qnorm_c;
+  } else {
+    const unsigned cblocks = (unsigned)ceil_div(ceil_div(n, 16), 4);
+    cand_kernel<kModeFP32><<<cblocks, 256, 0, st>>>(cp);
+    SG_LAUNCH_CHECK();
+  }
   finalize_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, st>>>(fp);
   SG_LAUNCH_CHECK();
   unsigned int nfail = 0, nfail2 = 0;
   SG_HIP(hipMemcpyAsync(&nfail, w.fail_count, 4, hipMemcpyDeviceToHost, st));
   SG_HIP(hipStreamSynchronize(st));
   if (nfail > 0 && !(flags & 1)) {
-    // second tier: exact FP64 distances for the failed rows, then the certificate again
+    // second tier: exact FP64 distances for the failed rows (database split into
+    // S segments so that few rows still fill the GPU), then the certificate again
     SG_HIP(hipMemsetAsync(w.fail_count2, 0, 4, st));
-    CandParams cp2 = cp;
-    cp2.rows = w.fail_rows;
-    cp2.n_rows = nfail;
-    cand_kernel<kModeExact><<<(unsigned)ceil_div(ceil_div((int64_t)nfail, 16), 4), 256, 0, st>>>(cp2);
+    const int64_t qwaves = ceil_div((int64_t)nfail, 16);
+    int S = (int)std::min<int64_t>(kMaxSeg, std::max<int64_t>(1, ceil_div(2048, qwaves)));
+    while (S > 1 && (int64_t)nfail * S > std::max<int64_t>(n, 2048)) --S;
+    const int64_t seg = ceil_div(ceil_div(n, S), 16) * 16;
+    SegParams sp{d_X, ld, n, d, w.fail_rows, (int64_t)nfail, seg, S, w.cand2, w.tau2};
+    exact_seg_kernel<<<dim3((unsigned)ceil_div(qwaves, 4), (unsigned)S), 256, 0, st>>>(sp);
     SG_LAUNCH_CHECK();
-    FinParams fp2 = fp;
-    fp2.rows = w.fail_rows;
-    fp2.n_rows = nfail;
-    fp2.mode = kModeExact;
-    fp2.fail_rows = w.fail_rows2;
-    fp2.fail_count = w.fail_count2;
-    finalize_kernel<<<(unsigned)ceil_div((int64_t)nfail, 4), 256, 0, st>>>(fp2);
+    finalize_multi_kernel<<<(unsigned)nfail, 256, 0, st>>>(fp, sp, w.fail_rows2, w.fail_count2);
     SG_LAUNCH_CHECK();
     SG_HIP(hipMemcpyAsync(&nfail2, w.fail_count2, 4, hipMemcpyDeviceToHost, st));
     SG_HIP(hipStreamSynchronize(st));
