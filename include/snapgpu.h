@@ -225,6 +225,25 @@ int snapgpu_spectral_apply(const int64_t* d_indptr, const int32_t* d_idx, const 
                            int64_t ldv, int32_t b, double* d_Y, int64_t ldy, int32_t value_type,
                            void* d_ws, int64_t ws_bytes, void* stream);
 
+/* Panel-segmented variant of snapgpu_spectral_apply for large problems: the
+ * dense operand is processed in panels of 32,768 rows (L2-resident), one launch
+ * per panel, with FP32 partial sums accumulated in panel order.
+ * snapgpu_spectral_segments(ptr, idx, nrows, dense_rows, seg) builds the table:
+ * seg[row * (P+1) + p] = offset of the row's first entry with idx >= p * 32768,
+ * P = snapgpu_spectral_panel_count(dense_rows). Call it on the CSR (dense rows =
+ * m) and on the CSC (dense rows = n). */
+int32_t snapgpu_spectral_panel_count(int64_t dense_rows);
+int snapgpu_spectral_segments(const int64_t* d_ptr, const int32_t* d_idx, int64_t nrows,
+                              int64_t ncols_dense, uint32_t* d_seg, void* stream);
+int snapgpu_spectral_apply_panels(const int64_t* d_indptr, const int32_t* d_idx,
+                                  const uint32_t* d_cnt, int64_t n, const int64_t* d_colptr,
+                                  const int32_t* d_rowidx, const uint32_t* d_ccnt, int64_t m,
+                                  const uint32_t* d_seg_csr, const uint32_t* d_seg_csc,
+                                  const float* d_r32, const float* d_w2, const double* d_dinv,
+                                  const double* d_V, int64_t ldv, int32_t b, double* d_Y,
+                                  int64_t ldy, int32_t value_type, void* d_ws, int64_t ws_bytes,
+                                  void* stream);
+
 /* FP64 tall-skinny products for the block Lanczos basis (the work ARPACK's
  * dsaupd does inside eigsh, embedding.rs:164). Row-major operands.
  * C[ka x kb] = A[n x ka]^T B[n x kb], fixed-order reduction; kb <= 32. */

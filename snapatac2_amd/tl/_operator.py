@@ -69,7 +69,7 @@ def select_columns(X: DeviceCSR, cols: np.ndarray | None) -> DeviceCSR:
 class SpectralOperator:
     """Holds CSR/CSC counts, weights, scales and degrees of A on the device."""
 
-    def __init__(self, X: DeviceCSR, feature_weights=None):
+    def __init__(self, X: DeviceCSR, feature_weights=None, use_panels: bool | None = None):
         torch = _lib.require_gpu()
         L = _lib.lib()
         self.torch = torch
@@ -116,6 +116,17 @@ class SpectralOperator:
             self.w.data_ptr(), self.s.data_ptr(), self.c.data_ptr(), self.dinv.data_ptr(),
             self.r64.data_ptr(), self.r32.data_ptr(), self.w2.data_ptr(), self.value_type,
             status.data_ptr(), st))
+        # panel segment tables (dense operands larger than one L2-sized panel)
+        pc = int(L.snapgpu_spectral_panel_count(m))
+        pr = int(L.snapgpu_spectral_panel_count(n))
+        self.use_panels = (pc > 1 or pr > 1) if use_panels is None else bool(use_panels)
+        if self.use_panels:
+            self.seg_csr = torch.empty(n * (pc + 1), dtype=torch.int32, device=dev)
+            self.seg_csc = torch.empty(m * (pr + 1), dtype=torch.int32, device=dev)
+            _lib.check(L.snapgpu_spectral_segments(X.indptr.data_ptr(), X.indices.data_ptr(), n, m,
+                                                   self.seg_csr.data_ptr(), st))
+            _lib.check(L.snapgpu_spectral_segments(self.colptr.data_ptr(), self.rowidx.data_ptr(),
+                                                   m, n, self.seg_csc.data_ptr(), st))
         self._ws_b = int(L.snapgpu_spectral_apply_workspace_bytes(n, m))
         self._ws = torch.empty(self._ws_b, dtype=torch.uint8, device=dev)
         self.n_applies = 0
@@ -135,12 +146,21 @@ class SpectralOperator:
         if self.events is not None:
             ev = (self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True))
             ev[0].record()
-        _lib.check(L.snapgpu_spectral_apply(
-            X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), self.n,
-            self.colptr.data_ptr(), self.rowidx.data_ptr(), self.cval.data_ptr(), self.m,
-            self.r32.data_ptr(), self.w2.data_ptr(), self.dinv.data_ptr(), V.data_ptr(),
-            V.stride(0), b, Y.data_ptr(), Y.stride(0), self.value_type, self._ws.data_ptr(),
-            self._ws_b, _lib.stream_ptr()))
+        if self.use_panels:
+            _lib.check(L.snapgpu_spectral_apply_panels(
+                X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), self.n,
+                self.colptr.data_ptr(), self.rowidx.data_ptr(), self.cval.data_ptr(), self.m,
+                self.seg_csr.data_ptr(), self.seg_csc.data_ptr(), self.r32.data_ptr(),
+                self.w2.data_ptr(), self.dinv.data_ptr(), V.data_ptr(), V.stride(0), b,
+                Y.data_ptr(), Y.stride(0), self.value_type, self._ws.data_ptr(), self._ws_b,
+                _lib.stream_ptr()))
+        else:
+            _lib.check(L.snapgpu_spectral_apply(
+                X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), self.n,
+                self.colptr.data_ptr(), self.rowidx.data_ptr(), self.cval.data_ptr(), self.m,
+                self.r32.data_ptr(), self.w2.data_ptr(), self.dinv.data_ptr(), V.data_ptr(),
+                V.stride(0), b, Y.data_ptr(), Y.stride(0), self.value_type, self._ws.data_ptr(),
+                self._ws_b, _lib.stream_ptr()))
         if ev is not None:
             ev[1].record()
             self.events.append(ev)

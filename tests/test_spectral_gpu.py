@@ -131,3 +131,27 @@ def test_errors(torch_gpu):
         snap.tl.spectral(ad, features=None, sample_size=1)
     with pytest.raises(NotImplementedError):
         snap.tl.spectral(ad, features=None, sample_size=0.5)
+
+
+def test_panel_segmented_operator(torch_gpu):
+    # more than one 32,768-row panel on both sides: 70,000 cells x 80,000 features
+    torch = torch_gpu
+    from snapatac2_amd.tl._operator import SpectralOperator, _to_device_csr
+    X = small_counts(70000, 80000, 0.0015, seed=12)
+    Xd = _to_device_csr(X, torch)
+    op_p = SpectralOperator(Xd, use_panels=True)
+    op_1 = SpectralOperator(Xd, use_panels=False)
+    Xh, dinv, _, _ = osp.operator_parts(X)
+    rng = np.random.default_rng(1)
+    for b in (16, 8, 32, 5):
+        V = rng.standard_normal((70000, b))
+        Vt = torch.from_numpy(V).cuda()
+        Yp = op_p.matmat(Vt).cpu().numpy()
+        Y1 = op_1.matmat(Vt).cpu().numpy()
+        ref = Xh @ (Xh.T @ V) - dinv[:, None] * V
+        scale = np.abs(ref).max()
+        assert np.abs(Yp - ref).max() / scale < 1e-5, b
+        assert np.abs(Yp - Y1).max() / scale < 1e-5, b
+    # bitwise reproducible
+    Vt = torch.from_numpy(rng.standard_normal((70000, 16))).cuda()
+    assert torch.equal(op_p.matmat(Vt), op_p.matmat(Vt))
