@@ -3,8 +3,11 @@
 Default workload (the configuration the headline metric is quoted on): 1,000,000
 synthetic cells (~6.9e9 fragments, generated on the device before timing),
 500 bp tiles with the default exclusions, top 500,000 features, spectral with 30
-eigenpairs (FP32 sparse products inside FP64 block Lanczos), exact kNN k=50 on
-the embedding (BF16 MFMA candidates + FP64 rescoring).
+eigenpairs (FP32 sparse products inside FP64 block Lanczos) with feature
+weights learned by one eigengap-ascent step from IDF through the hand-written
+adjoint, exact kNN k=50 on the embedding (BF16 MFMA candidates + FP64
+rescoring). The plain path (IDF weights, no ascent) is timed once more on the
+same input and reported beside it.
 
 A step = one pass of the whole pipeline over the resident fragments. Prints one
 JSON line (rank 0). Per-stage times go to stderr.
@@ -25,10 +28,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 WORKLOADS = {
-    # name: (cells, selected features, eigenpairs, k, knn precision)
-    "pipeline-1m": (1_000_000, 500_000, 30, 50, "bf16"),
-    "pipeline-100k": (100_000, 500_000, 30, 50, "fp32"),
-    "pipeline-2k": (2_000, 50_000, 30, 50, "bf16"),
+    # name: (cells, selected features, eigenpairs, k, knn precision, eigengap ascent steps)
+    "pipeline-1m": (1_000_000, 500_000, 30, 50, "bf16", 1),
+    "pipeline-100k": (100_000, 500_000, 30, 50, "fp32", 0),
+    "pipeline-2k": (2_000, 50_000, 30, 50, "bf16", 0),
 }
 HBM_PEAK_GBS = 8000.0
 EXCLUDE = ["chrM", "chrY", "M", "Y"]
@@ -59,15 +62,27 @@ class StageTimer:
         return _Ctx()
 
 
-def run_pipeline(torch, chunks, cfg, timer, spmm_events):
+def _embed(torch, evals, U, n_comps):
+    """Top n_comps eigenpairs by descending eigenvalue, positive ones scaled by
+    sqrt(lambda) (weighted_by_sd, snapatac2/tools/_embedding.py:273-281)."""
+    ix = np.argsort(evals, kind="stable")[::-1][:n_comps]
+    evals = evals[ix]
+    U = U[:, torch.from_numpy(ix.copy()).to(U.device)]
+    keep = np.flatnonzero(evals > 0)
+    return U[:, torch.from_numpy(keep).to(U.device)] * torch.from_numpy(
+        np.sqrt(evals[keep])).to(U.device), int(keep.size)
+
+
+def run_pipeline(torch, chunks, cfg, timer, spmm_events, learn_steps=0):
     from snapatac2_amd.genome import synthetic_genome
     from snapatac2_amd.pp import make_tile_matrix_chunked
     from snapatac2_amd.pp._basic import _column_sums, _find_most_accessible_features
     from snapatac2_amd.pp._knn import nearest_neighbour_graph
+    from snapatac2_amd.tl._adjoint import tune_feature_weights
     from snapatac2_amd.tl._lanczos import eigsh_lm
     from snapatac2_amd.tl._operator import SpectralOperator, select_columns
 
-    n_cells, n_feat, n_comps, k, prec = cfg
+    n_cells, n_feat, n_comps, k, prec = cfg[:5]
     cs = synthetic_genome()
     info = {}
     with timer("tile"):
@@ -84,18 +99,24 @@ def run_pipeline(torch, chunks, cfg, timer, spmm_events):
         op.events = spmm_events
     info["selected_nnz"] = Xs.nnz
     info["n_features"] = int(sel.size)
-    with timer("spectral_eigensolve"):
-        # n_comps = min(n_vars - 1, n_obs - 1, n_comps) (_embedding.py:231)
-        n_comps_eff = min(len(names) - 1, n_cells - 1, n_comps)
-        evals, U, st = eigsh_lm(op, n_comps_eff, seed=0)
-        ix = np.argsort(evals, kind="stable")[::-1]
-        evals = evals[ix]
-        U = U[:, torch.from_numpy(ix.copy()).to(U.device)]
-        keep = np.flatnonzero(evals > 0)
-        emb = U[:, torch.from_numpy(keep).to(U.device)] * torch.from_numpy(
-            np.sqrt(evals[keep])).to(U.device)
-    info["lanczos"] = st
-    info["n_eigen_kept"] = int(keep.size)
+    # n_comps = min(n_vars - 1, n_obs - 1, n_comps) (_embedding.py:231)
+    n_comps_eff = min(len(names) - 1, n_cells - 1, n_comps)
+    if learn_steps > 0:
+        with timer("spectral_learned_weights"):
+            # eigengap ascent on w from IDF; its last eigensolve (n_comps + 1 pairs)
+            # is the embedding's, for the learned weights
+            lanczos_before = op.n_applies
+            tr = tune_feature_weights(op, n_comps_eff, steps=learn_steps, seed=0)
+            emb, kept = _embed(torch, tr.evals, tr.vectors, n_comps_eff)
+        info["lanczos"] = {"applies": op.n_applies - lanczos_before, "block": 16}
+        info["eigengap"] = {"idf": tr.history[0][0], "learned": tr.history[-1][0],
+                            "accepted_steps": len(tr.history) - 1, "ascent_steps": learn_steps}
+    else:
+        with timer("spectral_eigensolve"):
+            evals, U, st = eigsh_lm(op, n_comps_eff, seed=0)
+            emb, kept = _embed(torch, evals, U, n_comps_eff)
+        info["lanczos"] = st
+    info["n_eigen_kept"] = kept
     with timer("knn"):
         g, kst = nearest_neighbour_graph(emb, k, precision=prec, return_stats=True)
     info["knn_fallback_rows"] = kst["fallback_rows"]
@@ -268,7 +289,8 @@ def main():
         info = emb = None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        info, emb, g = run_pipeline(torch, chunks, cfg, timer, spmm_events if timed else None)
+        info, emb, g = run_pipeline(torch, chunks, cfg, timer, spmm_events if timed else None,
+                                    learn_steps=cfg[5])
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         log(f"[bench] step {s} ({'timed' if timed else 'warmup'}): {dt:.3f}s stages "
@@ -279,6 +301,22 @@ def main():
                                  args.workload)
     ms = float(np.mean(times)) * 1e3
     value = n_cells / (ms / 1e3)
+    stages = {k: round(v, 3) for k, v in timer.t.items()}
+    plain = None
+    if cfg[5] > 0:
+        # the plain GPU path (IDF weights, no ascent) on the same resident input
+        ptimer = StageTimer(torch)
+        del info["op"]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pinfo, _, _ = run_pipeline(torch, chunks, cfg, ptimer, None, learn_steps=0)
+        torch.cuda.synchronize()
+        pdt = time.perf_counter() - t0
+        del pinfo["op"]
+        plain = {"ms_per_step": round(pdt * 1e3, 1), "cells_per_s": round(n_cells / pdt, 1),
+                 "stages_s": {k: round(v, 3) for k, v in ptimer.t.items()},
+                 "lanczos": pinfo["lanczos"]}
+        log(f"[bench] plain path (IDF weights): {pdt:.3f}s")
     log(f"[bench] lanczos {info['lanczos']}  nnz tile={info['tile_nnz']} sel={info['selected_nnz']}")
     # recall spot check of the kNN output against exact FP64 neighbours
     E = emb.cpu().numpy()
@@ -313,8 +351,10 @@ def main():
         "spmm_gbps": roof["achieved"] if roof else None,
         "knn_recall_sample": recall,
         "knn_fallback_rows": info["knn_fallback_rows"],
-        "stages_s": {k: round(v, 3) for k, v in timer.t.items()},
+        "stages_s": stages,
         "lanczos": info["lanczos"],
+        "eigengap": info.get("eigengap"),
+        "plain_path": plain,
     }
     if cpu:
         out["vs_cpu"] = round(value / cpu["value"], 1)

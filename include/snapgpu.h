@@ -225,6 +225,22 @@ int snapgpu_spectral_apply(const int64_t* d_indptr, const int32_t* d_idx, const 
                            int64_t ldv, int32_t b, double* d_Y, int64_t ldy, int32_t value_type,
                            void* d_ws, int64_t ws_bytes, void* stream);
 
+/* Adjoint with respect to the feature weights (no counterpart in the
+ * reference; it differentiates the operator above, embedding.rs:133-162 with
+ * the weighting of embedding.rs:314-326): dw = d<G, A(w) V>/dw, including the
+ * row-norm and degree dependence on w. d_s, d_dinv, d_r32, d_w2 are the
+ * outputs of snapgpu_spectral_prepare for the same d_w. The derivative with
+ * respect to V is A G (A is symmetric): call snapgpu_spectral_apply on G.
+ * Workspace: snapgpu_spectral_vjp_workspace_bytes(n, m). */
+int64_t snapgpu_spectral_vjp_workspace_bytes(int64_t n, int64_t m);
+int snapgpu_spectral_vjp_w(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_cnt,
+                           int64_t n, const int64_t* d_colptr, const int32_t* d_rowidx,
+                           const uint32_t* d_ccnt, int64_t m, const double* d_w,
+                           const double* d_s, const double* d_dinv, const float* d_r32,
+                           const float* d_w2, const double* d_V, int64_t ldv, const double* d_G,
+                           int64_t ldg, int32_t b, double* d_dw, int32_t value_type, void* d_ws,
+                           int64_t ws_bytes, void* stream);
+
 /* Panel-segmented variant of snapgpu_spectral_apply for large problems: the
  * dense operand is processed in panels of 32,768 rows (L2-resident), one launch
  * per panel, with FP32 partial sums accumulated in panel order.

@@ -48,6 +48,9 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_spectral_apply_workspace_bytes": (_i64, [_i64, _i64]),
     "snapgpu_spectral_apply": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _p,
                                          _i64, _i32, _p, _i64, _i32, _p, _i64, _p]),
+    "snapgpu_spectral_vjp_workspace_bytes": (_i64, [_i64, _i64]),
+    "snapgpu_spectral_vjp_w": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _p, _p,
+                                         _p, _i64, _p, _i64, _i32, _p, _i32, _p, _i64, _p]),
     "snapgpu_spectral_panel_count": (_i32, [_i64]),
     "snapgpu_spectral_segments": (C.c_int, [_p, _p, _i64, _i64, _p, _p]),
     "snapgpu_spectral_apply_panels": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _p,

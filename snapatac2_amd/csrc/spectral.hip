@@ -306,7 +306,7 @@ __global__ void __launch_bounds__(256) spmm_kernel(const int64_t* __restrict__ p
   for (int o = BW; o < 64; o <<= 1) acc += __shfl_xor(acc, o, 64);
   if (g != 0) return;
   if (EPI == 0) {
-    out32[row * BW + c] = acc * rowscale[row];
+    out32[row * BW + c] = rowscale ? acc * rowscale[row] : acc;
   } else if (c < b) {
     Y[row * ldy + c] = (double)(acc * rowscale[row]) - dinv[row] * V[row * ldv + c];
   }
@@ -570,6 +570,175 @@ __global__ void random_kernel(uint64_t seed, int64_t n, double* out) {
   out[i] = ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0) - 0.5;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the operator with respect to the feature weights. With u = row
+// scales, d = degrees, a = d^-1/2, r = u a, e = X^T u, P = X^T (r o V) and
+// Q = X^T (r o G), the derivative of f = <G, A(w) V> is
+//   h_j   = <P_j, Q_j>
+//   alpha = rowdot(G, X (w^2 o P)) + rowdot(V, X (w^2 o Q)),  beta = rowdot(G, V)
+//   gamma = -alpha u d^-3/2 / 2 + beta d^-2
+//   phi   = X^T (gamma o u),  psi = X (w^2 o phi)
+//   kappa = alpha a u^3 + gamma u^2 (d + 1) + u^3 psi,  sigma = (X o X)^T kappa
+//   df/dw = w o (2 h + 2 e o phi - sigma)
+// covering the explicit w^2, the row norms u(w) and the degrees d(w). X W is
+// never formed: every term is one pass over the stored counts.
+// ---------------------------------------------------------------------------
+__global__ void rowdot_f32_kernel(const float* A, const float* B, int64_t rows, int BW, int b,
+                                  double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  double s = 0.0;
+  for (int c = 0; c < b; ++c) s += (double)A[i * BW + c] * (double)B[i * BW + c];
+  out[i] = s;
+}
+
+__global__ void scale_rows_f32_kernel(float* A, int64_t rows, int BW, const float* sc) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= rows * BW) return;
+  A[t] *= sc[t / BW];
+}
+
+// alpha, gamma and t = gamma u per cell (all zero for an empty row)
+__global__ void adj_rows_kernel(const double* G, int64_t ldg, const double* V, int64_t ldv,
+                                const float* XP, const float* XQ, int BW, int b, int64_t n,
+                                const double* u, const double* dinv, double* alpha, double* gamma,
+                                double* t) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double al = 0.0, be = 0.0;
+  for (int c = 0; c < b; ++c) {
+    const double g = G[i * ldg + c], v = V[i * ldv + c];
+    al += g * (double)XP[i * BW + c] + v * (double)XQ[i * BW + c];
+    be += g * v;
+  }
+  double ga = 0.0;
+  if (u[i] > 0.0) ga = -0.5 * al * u[i] * dinv[i] * sqrt(dinv[i]) + be * dinv[i] * dinv[i];
+  alpha[i] = al;
+  gamma[i] = ga;
+  t[i] = ga * u[i];
+}
+
+// y1_j = sum_i cnt_ij^POW x1_i (and y2 from x2 when given); one wave per column.
+template <bool FV, int POW>
+__global__ void __launch_bounds__(256) csc_spmv2_kernel(const int64_t* colptr, const int32_t* rowidx,
+                                                        const uint32_t* cnt, int64_t m, const double* x1,
+                                                        const double* x2, double* y1, double* y2) {
+  const int lane = threadIdx.x & 63;
+  const int64_t col = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (col >= m) return;
+  double a1 = 0.0, a2 = 0.0;
+  for (int64_t e = colptr[col] + lane; e < colptr[col + 1]; e += 64) {
+    double v = val64<FV>(cnt, e);
+    if (POW == 2) v *= v;
+    const int32_t i = rowidx[e];
+    a1 = fma(v, x1[i], a1);
+    if (x2 != nullptr) a2 = fma(v, x2[i], a2);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a1 += __shfl_xor(a1, o, 64);
+    a2 += __shfl_xor(a2, o, 64);
+  }
+  if (lane == 0) {
+    y1[col] = a1;
+    if (y2 != nullptr) y2[col] = a2;
+  }
+}
+
+// y_i = sum_j cnt_ij w_j^2 x_j; one wave per row.
+template <bool FV>
+__global__ void __launch_bounds__(256) csr_spmv_w2_kernel(const int64_t* indptr, const int32_t* idx,
+                                                          const uint32_t* cnt, int64_t n, const double* w,
+                                                          const double* x, double* y) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (row >= n) return;
+  double a = 0.0;
+  for (int64_t e = indptr[row] + lane; e < indptr[row + 1]; e += 64) {
+    const int32_t j = idx[e];
+    a = fma(val64<FV>(cnt, e) * (w[j] * w[j]), x[j], a);
+  }
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+  if (lane == 0) y[row] = a;
+}
+
+__global__ void kappa_kernel(int64_t n, const double* u, const double* dinv, const double* alpha,
+                             const double* gamma, const double* psi, double* kappa) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double ui = u[i];
+  double k = 0.0;
+  if (ui > 0.0) {
+    const double u2 = ui * ui, u3 = u2 * ui;
+    k = alpha[i] * sqrt(dinv[i]) * u3 + gamma[i] * u2 * (1.0 / dinv[i] + 1.0) + u3 * psi[i];
+  }
+  kappa[i] = k;
+}
+
+__global__ void dw_kernel(int64_t m, const double* w, const double* h, const double* e,
+                          const double* phi, const double* sigma, double* dw) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  dw[j] = w[j] * (2.0 * h[j] + 2.0 * e[j] * phi[j] - sigma[j]);
+}
+
+// Workspace of the adjoint: two n x 32 and two m x 32 FP32 blocks, then
+// 5 n-vectors and 4 m-vectors in FP64, each 256-byte aligned.
+static int64_t vjp_ws_bytes(int64_t n, int64_t m) {
+  auto al = [](int64_t b) { return (b + 255) / 256 * 256; };
+  return 2 * al(n * kB * 4) + 2 * al(m * kB * 4) + 5 * al(n * 8) + 4 * al(m * 8) + 256;
+}
+
+template <int BW, bool FV>
+static void launch_vjp(const int64_t* indptr, const int32_t* idx, const uint32_t* cnt, int64_t n,
+                       const int64_t* colptr, const int32_t* rowidx, const uint32_t* ccnt, int64_t m,
+                       const double* u, const double* dinv, const double* w, const float* r32,
+                       const float* w2, const double* V, int64_t ldv, const double* G, int64_t ldg,
+                       int b, double* dw, void* ws, hipStream_t st) {
+  char* q = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  auto take = [&q](int64_t bytes) {
+    char* p = q;
+    q += (bytes + 255) / 256 * 256;
+    return p;
+  };
+  float* Vs = (float*)take(n * kB * 4);
+  float* Gs = (float*)take(n * kB * 4);
+  float* Pm = (float*)take(m * kB * 4);
+  float* Qm = (float*)take(m * kB * 4);
+  double* alpha = (double*)take(n * 8);
+  double* gamma = (double*)take(n * 8);
+  double* t = (double*)take(n * 8);
+  double* psi = (double*)take(n * 8);
+  double* kappa = (double*)take(n * 8);
+  double* h = (double*)take(m * 8);
+  double* e = (double*)take(m * 8);
+  double* phi = (double*)take(m * 8);
+  double* sigma = (double*)take(m * 8);
+  const unsigned gn = (unsigned)ceil_div(n, 4), gm = (unsigned)ceil_div(m, 4);
+  scale_in_kernel<BW><<<(unsigned)ceil_div(n * BW, 256), 256, 0, st>>>(V, ldv, n, b, r32, Vs);
+  scale_in_kernel<BW><<<(unsigned)ceil_div(n * BW, 256), 256, 0, st>>>(G, ldg, n, b, r32, Gs);
+  // P = X^T (r o V), Q = X^T (r o G)
+  spmm_kernel<BW, 0, FV><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, Vs, nullptr, Pm, nullptr, 0,
+                                             nullptr, nullptr, 0, b);
+  spmm_kernel<BW, 0, FV><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, Gs, nullptr, Qm, nullptr, 0,
+                                             nullptr, nullptr, 0, b);
+  rowdot_f32_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(Pm, Qm, m, BW, b, h);
+  scale_rows_f32_kernel<<<(unsigned)ceil_div(m * BW, 256), 256, 0, st>>>(Pm, m, BW, w2);
+  scale_rows_f32_kernel<<<(unsigned)ceil_div(m * BW, 256), 256, 0, st>>>(Qm, m, BW, w2);
+  // X (w^2 o P) into Vs and X (w^2 o Q) into Gs (both free after the CSC pass)
+  spmm_kernel<BW, 0, FV><<<gn, 256, 0, st>>>(indptr, idx, cnt, n, Pm, nullptr, Vs, nullptr, 0,
+                                             nullptr, nullptr, 0, b);
+  spmm_kernel<BW, 0, FV><<<gn, 256, 0, st>>>(indptr, idx, cnt, n, Qm, nullptr, Gs, nullptr, 0,
+                                             nullptr, nullptr, 0, b);
+  adj_rows_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(G, ldg, V, ldv, Vs, Gs, BW, b, n, u, dinv,
+                                                              alpha, gamma, t);
+  // phi = X^T (gamma o u) and e = X^T u in one pass over the CSC
+  csc_spmv2_kernel<FV, 1><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, t, u, phi, e);
+  csr_spmv_w2_kernel<FV><<<gn, 256, 0, st>>>(indptr, idx, cnt, n, w, phi, psi);
+  kappa_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(n, u, dinv, alpha, gamma, psi, kappa);
+  csc_spmv2_kernel<FV, 2><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, kappa, nullptr, sigma, nullptr);
+  dw_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(m, w, h, e, phi, sigma, dw);
+}
+
 }  // namespace spectral
 }  // namespace snapgpu
 
@@ -774,6 +943,39 @@ int snapgpu_spectral_apply(const int64_t* d_indptr, const int32_t* d_idx, const 
     SG_APPLY(32)
   }
 #undef SG_APPLY
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
+int64_t snapgpu_spectral_vjp_workspace_bytes(int64_t n, int64_t m) { return vjp_ws_bytes(n, m); }
+
+int snapgpu_spectral_vjp_w(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_cnt, int64_t n,
+                           const int64_t* d_colptr, const int32_t* d_rowidx, const uint32_t* d_ccnt,
+                           int64_t m, const double* d_w, const double* d_s, const double* d_dinv,
+                           const float* d_r32, const float* d_w2, const double* d_V, int64_t ldv,
+                           const double* d_G, int64_t ldg, int32_t b, double* d_dw, int32_t value_type,
+                           void* d_ws, int64_t ws_bytes, void* stream) {
+  clear_error();
+  SG_ARG(n >= 1 && m >= 1, "bad shape");
+  SG_ARG(b >= 1 && b <= kB, "block width must be in [1, 32]");
+  SG_ARG(value_type == 0 || value_type == 1, "value_type must be 0 (uint32) or 1 (float32)");
+  SG_ARG(ldv >= b && ldg >= b, "leading dimension < b");
+  SG_ARG(d_ws != nullptr && ws_bytes >= vjp_ws_bytes(n, m), "workspace too small");
+  hipStream_t st = as_stream(stream);
+  const bool fv = value_type == 1;
+#define SG_VJP(BW)                                                                                   \
+  if (fv) launch_vjp<BW, true>(d_indptr, d_idx, d_cnt, n, d_colptr, d_rowidx, d_ccnt, m, d_s, d_dinv, \
+                               d_w, d_r32, d_w2, d_V, ldv, d_G, ldg, b, d_dw, d_ws, st);            \
+  else launch_vjp<BW, false>(d_indptr, d_idx, d_cnt, n, d_colptr, d_rowidx, d_ccnt, m, d_s, d_dinv,   \
+                             d_w, d_r32, d_w2, d_V, ldv, d_G, ldg, b, d_dw, d_ws, st);
+  if (b <= 8) {
+    SG_VJP(8)
+  } else if (b <= 16) {
+    SG_VJP(16)
+  } else {
+    SG_VJP(32)
+  }
+#undef SG_VJP
   SG_LAUNCH_CHECK();
   return SNAPGPU_OK;
 }

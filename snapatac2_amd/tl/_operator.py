@@ -92,30 +92,23 @@ class SpectralOperator:
                                            chunk, self.colptr.data_ptr(), self.rowidx.data_ptr(),
                                            self.cval.data_ptr(), ws.data_ptr(), ws_b, st))
         del ws
+        f64 = dict(dtype=torch.float64, device=dev)
         if feature_weights is None:
-            self.w = torch.empty(m, dtype=torch.float64, device=dev)
+            self.w = torch.empty(m, **f64)
             scratch = torch.empty(16, dtype=torch.uint8, device=dev)
             _lib.check(L.snapgpu_spectral_idf(self.colptr.data_ptr(), m, n, self.w.data_ptr(),
                                               scratch.data_ptr(), st))
         else:
-            w = np.asarray(feature_weights, dtype=np.float64)
-            if w.shape != (m,):
-                raise ValueError(f"feature_weights has {w.size} entries, expected {m}")
-            self.w = torch.from_numpy(w).to(dev)
-        f64 = dict(dtype=torch.float64, device=dev)
+            self.w = torch.empty(m, **f64)
+            self._load_weights(feature_weights)
         self.s = torch.empty(n, **f64)
         self.c = torch.empty(m, **f64)
         self.dinv = torch.empty(n, **f64)
         self.r64 = torch.empty(n, **f64)
         self.r32 = torch.empty(n, dtype=torch.float32, device=dev)
         self.w2 = torch.empty(m, dtype=torch.float32, device=dev)
-        status = torch.zeros(8, dtype=torch.int32, device=dev)
-        _lib.check(L.snapgpu_spectral_prepare(
-            X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), n,
-            self.colptr.data_ptr(), self.rowidx.data_ptr(), self.cval.data_ptr(), m,
-            self.w.data_ptr(), self.s.data_ptr(), self.c.data_ptr(), self.dinv.data_ptr(),
-            self.r64.data_ptr(), self.r32.data_ptr(), self.w2.data_ptr(), self.value_type,
-            status.data_ptr(), st))
+        self._status = torch.zeros(8, dtype=torch.int32, device=dev)
+        self._prepare()
         # panel segment tables (dense operands larger than one L2-sized panel)
         pc = int(L.snapgpu_spectral_panel_count(m))
         pr = int(L.snapgpu_spectral_panel_count(n))
@@ -132,6 +125,61 @@ class SpectralOperator:
         self.n_applies = 0
         self.n_columns = 0
         self.events = None  # optional list collecting (start, end) events per application
+        self._vjp_ws = None
+
+    def _load_weights(self, w) -> None:
+        torch = self.torch
+        if isinstance(w, torch.Tensor):
+            w = w.detach().to(device=self.w.device, dtype=torch.float64).reshape(-1)
+        else:
+            w = torch.from_numpy(np.ascontiguousarray(w, dtype=np.float64).reshape(-1)).to(self.w.device)
+        if w.numel() != self.m:
+            raise ValueError(f"feature_weights has {w.numel()} entries, expected {self.m}")
+        self.w.copy_(w)
+
+    def _prepare(self) -> None:
+        """Row scales, column sums, degrees and r, w^2 for the current weights."""
+        L = _lib.lib()
+        X = self.X
+        _lib.check(L.snapgpu_spectral_prepare(
+            X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), self.n,
+            self.colptr.data_ptr(), self.rowidx.data_ptr(), self.cval.data_ptr(), self.m,
+            self.w.data_ptr(), self.s.data_ptr(), self.c.data_ptr(), self.dinv.data_ptr(),
+            self.r64.data_ptr(), self.r32.data_ptr(), self.w2.data_ptr(), self.value_type,
+            self._status.data_ptr(), _lib.stream_ptr()))
+
+    def set_weights(self, w) -> None:
+        """Replace the feature weights and recompute scales and degrees on the device."""
+        self._load_weights(w)
+        self._prepare()
+
+    def vjp_w(self, V, G):
+        """dw = d<G, A(w) V>/dw (FP64 device vector of length m) for FP64 n x b blocks."""
+        torch = self.torch
+        L = _lib.lib()
+        X = self.X
+        if V.shape != G.shape or V.shape[0] != self.n:
+            raise ValueError("V and G must both be n x b")
+        if V.stride(1) != 1 or G.stride(1) != 1:
+            raise ValueError("V and G must be row-major")
+        if self._vjp_ws is None:
+            self._vjp_ws_b = int(L.snapgpu_spectral_vjp_workspace_bytes(self.n, self.m))
+            self._vjp_ws = torch.empty(self._vjp_ws_b, dtype=torch.uint8, device=self.w.device)
+        dw = torch.zeros(self.m, dtype=torch.float64, device=self.w.device)
+        part = torch.empty_like(dw) if V.shape[1] > 32 else dw
+        for c0 in range(0, V.shape[1], 32):
+            b = min(32, V.shape[1] - c0)
+            Vc, Gc = V[:, c0:c0 + b], G[:, c0:c0 + b]
+            _lib.check(L.snapgpu_spectral_vjp_w(
+                X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), self.n,
+                self.colptr.data_ptr(), self.rowidx.data_ptr(), self.cval.data_ptr(), self.m,
+                self.w.data_ptr(), self.s.data_ptr(), self.dinv.data_ptr(), self.r32.data_ptr(),
+                self.w2.data_ptr(), Vc.data_ptr(), Vc.stride(0), Gc.data_ptr(), Gc.stride(0), b,
+                part.data_ptr(), self.value_type, self._vjp_ws.data_ptr(), self._vjp_ws_b,
+                _lib.stream_ptr()))
+            if part is not dw:
+                dw += part
+        return dw
 
     def apply(self, V, Y, b: int | None = None) -> None:
         """Y[:, :b] = A V[:, :b]; V, Y row-major FP64 device tensors (any row stride)."""
