@@ -120,22 +120,29 @@ def run_pipeline(torch, chunks, cfg, timer, spmm_events, learn_steps=0):
     with timer("knn"):
         g, kst = nearest_neighbour_graph(emb, k, precision=prec, return_stats=True)
     info["knn_fallback_rows"] = kst["fallback_rows"]
+    info["knn_chunk_visit_frac"] = (round(kst["chunk_visits"] / kst["chunk_pairs"], 4)
+                                    if kst.get("chunk_pairs") else None)
     info["op"] = op
     return info, emb, g
 
 
-def spmm_algorithmic_bytes(n, m, nnz, b):
-    """Algorithmic bytes of one operator application on a block of b columns
-    (b-wide FP32 dense rows; int32 index + uint32 count per stored entry; int64
-    row pointers). Every array is counted once:
-      CSC pass: colptr, row ids + counts, gathered Vs (n x b), Z written (m x b), w^2
-      CSR pass: indptr, column ids + counts, gathered Z (m x b), r, dinv, V read, Y written
-      scale:    V read, r, Vs written."""
-    f32, f64 = 4, 8
-    csc = (m + 1) * 8 + nnz * 8 + n * b * f32 + m * b * f32 + m * f32
-    csr = (n + 1) * 8 + nnz * 8 + m * b * f32 + n * f32 + n * f64 + n * b * f64 + n * b * f64
-    scale = n * b * f64 + n * f32 + n * b * f32
-    return csc + csr + scale
+def spmm_algorithmic_bytes(op, b):
+    """Algorithmic bytes of one operator application on a block of b columns,
+    in the form 2 [nnz e + pointers] + 4b (2n + 2m) + 8n: the canonical CSR
+    figure with e = the index + value bytes actually read per entry (4 for the
+    packed panel layout, 8 for separate index and count arrays) and, as the
+    pointer arrays, whatever the layout reads instead of indptr (chunk tables
+    and row -> pair lists for the panel layout, int64 row pointers otherwise).
+    V read, Z written and read, Y written in b-wide rows; dinv and r per cell."""
+    n, m, nnz = op.n, op.m, op.X.nnz
+    if op.use_panels:
+        ptrs = 0
+        e = 0
+        for L in (op.panel_csr, op.panel_csc):
+            e = max(e, 4 if L.packed else 8)
+            ptrs += (L.n_chunks + 1) * 16 + (L.n_rows + 1) * 8 + L.n_pairs * 4
+        return 2 * nnz * e + ptrs + 4 * b * (2 * n + 2 * m) + 8 * n
+    return 2 * (nnz * 8 + (n + 1) * 8) + 4 * b * (2 * n + 2 * m) + 8 * n
 
 
 def spmm_roofline(torch, op, events, block, workload):
@@ -145,7 +152,7 @@ def spmm_roofline(torch, op, events, block, workload):
         return None
     durs = [a.elapsed_time(b) for a, b in events]
     avg_ms = float(np.mean(durs))
-    alg = spmm_algorithmic_bytes(op.n, op.m, op.X.nnz, block)
+    alg = spmm_algorithmic_bytes(op, block)
     gbs = alg / (avg_ms * 1e-3) / 1e9
     traffic = None
     tp = os.path.join(ROOT, "profiles", f"spmm_pair_{workload}_traffic.json")
@@ -154,7 +161,8 @@ def spmm_roofline(torch, op, events, block, workload):
             traffic = int(json.load(f)["traffic_bytes_per_application"])
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": f"spectral operator application (scale + CSC SpMM + CSR SpMM, {block} columns)",
+            "kernel": (f"spectral operator application ({'panel layout: ' if op.use_panels else ''}"
+                       f"scale + CSC product + CSR product, {block} columns)"),
             "avg_ms": round(avg_ms, 3), "launches": len(durs), "alg_bytes": int(alg)}
 
 
@@ -351,6 +359,7 @@ def main():
         "spmm_gbps": roof["achieved"] if roof else None,
         "knn_recall_sample": recall,
         "knn_fallback_rows": info["knn_fallback_rows"],
+        "knn_chunk_visit_frac": info["knn_chunk_visit_frac"],
         "stages_s": stages,
         "lanczos": info["lanczos"],
         "eigengap": info.get("eigengap"),

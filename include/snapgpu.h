@@ -165,11 +165,16 @@ int64_t snapgpu_knn_workspace_bytes(int64_t n, int32_t d, int32_t k);
  * Outputs: d_nbr[n*k] (int32) and d_dist[n*k] (FP64 Euclidean distance), each
  * row sorted by neighbour index (knn.rs:67). Rows whose certificate fails are
  * screened again with exact FP64 distances (*n_fallback counts them) and, if
- * that certificate fails too, scanned exactly (*n_exact). flags bit 0: send
- * every row straight to the exact scan (testing). Synchronises `stream`. */
+ * that certificate fails too, scanned exactly (*n_exact). Mode 1 sorts the
+ * points spatially and skips 128-point chunks whose bounding sphere is provably
+ * farther than every candidate threshold of a 128-query workgroup;
+ * *n_chunk_visits (may be NULL) receives the (workgroup, chunk) pairs processed,
+ * out of ceil(n/128)^2. flags bit 0: send every row straight to the exact scan
+ * (testing); bit 1: visit every chunk (no skipping). Synchronises `stream`. */
 int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, int32_t mode,
                 int32_t* d_nbr, double* d_dist, void* d_workspace, int64_t workspace_bytes,
-                int32_t flags, int64_t* n_fallback, int64_t* n_exact, void* stream);
+                int32_t flags, int64_t* n_fallback, int64_t* n_exact, int64_t* n_chunk_visits,
+                void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Path B: the normalized-cosine operator inside spectral()                  */
@@ -241,24 +246,55 @@ int snapgpu_spectral_vjp_w(const int64_t* d_indptr, const int32_t* d_idx, const 
                            int64_t ldg, int32_t b, double* d_dw, int32_t value_type, void* d_ws,
                            int64_t ws_bytes, void* stream);
 
-/* Panel-segmented variant of snapgpu_spectral_apply for large problems: the
- * dense operand is processed in panels of 32,768 rows (L2-resident), one launch
- * per panel, with FP32 partial sums accumulated in panel order.
- * snapgpu_spectral_segments(ptr, idx, nrows, dense_rows, seg) builds the table:
- * seg[row * (P+1) + p] = offset of the row's first entry with idx >= p * 32768,
- * P = snapgpu_spectral_panel_count(dense_rows). Call it on the CSR (dense rows =
- * m) and on the CSC (dense rows = n). */
-int32_t snapgpu_spectral_panel_count(int64_t dense_rows);
-int snapgpu_spectral_segments(const int64_t* d_ptr, const int32_t* d_idx, int64_t nrows,
-                              int64_t ncols_dense, uint32_t* d_seg, void* stream);
-int snapgpu_spectral_apply_panels(const int64_t* d_indptr, const int32_t* d_idx,
-                                  const uint32_t* d_cnt, int64_t n, const int64_t* d_colptr,
-                                  const int32_t* d_rowidx, const uint32_t* d_ccnt, int64_t m,
-                                  const uint32_t* d_seg_csr, const uint32_t* d_seg_csc,
-                                  const float* d_r32, const float* d_w2, const double* d_dinv,
-                                  const double* d_V, int64_t ldv, int32_t b, double* d_Y,
-                                  int64_t ldy, int32_t value_type, void* d_ws, int64_t ws_bytes,
-                                  void* stream);
+/* Panel layout for large operators (the same products as snapgpu_spectral_apply).
+ * The matrix (CSR of the cells or CSC of the features) is re-laid out
+ * panel-major: panel p holds the entries whose column lies in
+ * [p * 32768, (p + 1) * 32768), so each product gathers from an L2-sized slice
+ * of the dense operand; work is split into chunks of ~1024 entries aligned to
+ * (row, panel) segments ("pairs"), whose FP32 sums are then added per row in
+ * panel order. Counts below 65536 are packed with the column into 4 bytes.
+ * Build: snapgpu_panel_sizes, then allocate, then snapgpu_panel_build with the
+ * same workspace (sizes leaves its tables there).
+ *   ent / val       n_entries uint32 (val only when not packed)
+ *   chunk_start     n_chunks + 1 int64;   chunk_pair n_chunks + 1 int32
+ *   chunk_panel     n_chunks int32
+ *   row_pair_ptr    n_rows + 1 int64;     row_pair_ids n_pairs int32 */
+typedef struct snapgpu_panel_layout {
+  const uint32_t* ent;
+  const uint32_t* val;
+  const int64_t* chunk_start;
+  const int32_t* chunk_pair;
+  const int32_t* chunk_panel;
+  int64_t n_chunks;
+  const int64_t* row_pair_ptr;
+  const int32_t* row_pair_ids;
+  int64_t n_pairs;
+  int64_t n_rows;
+  int64_t dense_rows;
+  int32_t packed;
+  int32_t value_type;
+} snapgpu_panel_layout;
+
+int32_t snapgpu_panel_count(int64_t dense_rows);
+int64_t snapgpu_panel_build_workspace_bytes(int64_t nrows, int64_t dense_rows, int64_t nnz);
+int snapgpu_panel_sizes(const int64_t* d_ptr, const int32_t* d_idx, const uint32_t* d_val,
+                        int64_t nrows, int64_t dense_rows, int64_t nnz, int32_t value_type,
+                        void* d_ws, int64_t ws_bytes, int64_t* n_pairs, int64_t* n_chunks,
+                        int32_t* packed, void* stream);
+int snapgpu_panel_build(const int64_t* d_ptr, const int32_t* d_idx, const uint32_t* d_val,
+                        int64_t nrows, int64_t dense_rows, int64_t nnz, int32_t packed, void* d_ws,
+                        int64_t ws_bytes, uint32_t* d_ent, uint32_t* d_val_out,
+                        int64_t* d_chunk_start, int32_t* d_chunk_pair, int32_t* d_chunk_panel,
+                        int64_t n_chunks, int64_t* d_row_pair_ptr, int32_t* d_row_pair_ids,
+                        void* stream);
+/* Y = A V as snapgpu_spectral_apply, over the two panel layouts (csr: rows =
+ * cells, dense_rows = features; csc: rows = features, dense_rows = cells). */
+int64_t snapgpu_spectral_apply_panel_workspace_bytes(int64_t n, int64_t m, int64_t pairs_csr,
+                                                     int64_t pairs_csc);
+int snapgpu_spectral_apply_panel(const snapgpu_panel_layout* csr, const snapgpu_panel_layout* csc,
+                                 const float* d_r32, const float* d_w2, const double* d_dinv,
+                                 const double* d_V, int64_t ldv, int32_t b, double* d_Y,
+                                 int64_t ldy, void* d_ws, int64_t ws_bytes, void* stream);
 
 /* FP64 tall-skinny products for the block Lanczos basis (the work ARPACK's
  * dsaupd does inside eigsh, embedding.rs:164). Row-major operands.

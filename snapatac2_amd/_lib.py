@@ -36,7 +36,7 @@ SIGNATURES: dict[str, tuple] = {
                                           _i32, C.c_double, C.c_double, _p, _p, _p, _i64, _p]),
     "snapgpu_knn_workspace_bytes": (_i64, [_i64, _i32, _i32]),
     "snapgpu_knn": (C.c_int, [_p, _i64, _i32, _i64, _i32, _i32, _p, _p, _p, _i64, _i32,
-                              C.POINTER(_i64), C.POINTER(_i64), _p]),
+                              C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64), _p]),
     "snapgpu_csr_select_count": (C.c_int, [_p, _p, _i32, _i64, _p, _p, _p]),
     "snapgpu_csr_select_fill": (C.c_int, [_p, _p, _i32, _p, _i64, _p, _p, _p, _p, _p]),
     "snapgpu_csr_transpose_workspace_bytes": (_i64, [_i64, _i64]),
@@ -51,16 +51,29 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_spectral_vjp_workspace_bytes": (_i64, [_i64, _i64]),
     "snapgpu_spectral_vjp_w": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _p, _p,
                                          _p, _i64, _p, _i64, _i32, _p, _i32, _p, _i64, _p]),
-    "snapgpu_spectral_panel_count": (_i32, [_i64]),
-    "snapgpu_spectral_segments": (C.c_int, [_p, _p, _i64, _i64, _p, _p]),
-    "snapgpu_spectral_apply_panels": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _p,
-                                                _p, _p, _i64, _i32, _p, _i64, _i32, _p, _i64, _p]),
+    "snapgpu_panel_count": (_i32, [_i64]),
+    "snapgpu_panel_build_workspace_bytes": (_i64, [_i64, _i64, _i64]),
+    "snapgpu_panel_sizes": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _i32, _p, _i64,
+                                      C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i32), _p]),
+    "snapgpu_panel_build": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _i32, _p, _i64, _p, _p, _p, _p,
+                                      _p, _i64, _p, _p, _p]),
+    "snapgpu_spectral_apply_panel_workspace_bytes": (_i64, [_i64, _i64, _i64, _i64]),
+    "snapgpu_spectral_apply_panel": (C.c_int, [_p, _p, _p, _p, _p, _p, _i64, _i32, _p, _i64, _p,
+                                               _i64, _p]),
     "snapgpu_gemm_tn_workspace_bytes": (_i64, [_i64, _i32, _i32]),
     "snapgpu_gemm_tn_f64": (C.c_int, [_p, _i64, _p, _i64, _i64, _i32, _i32, _p, _p, _i64, _p]),
     "snapgpu_gemm_nn_f64": (C.c_int, [_p, _i64, _p, _i64, _i32, _i32, C.c_double, C.c_double,
                                       _p, _i64, _p]),
     "snapgpu_random_f64": (C.c_int, [_u64, _i64, _p, _p]),
 }
+
+
+class PanelLayout(C.Structure):
+    """snapgpu_panel_layout (include/snapgpu.h): device pointers and sizes."""
+    _fields_ = [("ent", _p), ("val", _p), ("chunk_start", _p), ("chunk_pair", _p),
+                ("chunk_panel", _p), ("n_chunks", _i64), ("row_pair_ptr", _p),
+                ("row_pair_ids", _p), ("n_pairs", _i64), ("n_rows", _i64), ("dense_rows", _i64),
+                ("packed", _i32), ("value_type", _i32)]
 
 
 class SnapGpuError(RuntimeError):

@@ -29,6 +29,7 @@
 //     fallback kernel.
 //  4. output: k neighbours per row sorted by index, distance = sqrt(d^2).
 #include <cfloat>
+#include <vector>
 
 #include <hipcub/hipcub.hpp>
 
@@ -138,18 +139,23 @@ __device__ __forceinline__ void list_insert(float (&L)[kList], int32_t (&I)[kLis
 }
 
 // ---------------------------------------------------------------------------
-// 2b. spatial query order + per-workgroup centres (BF16 path)
+// 2b. spatial order + per-workgroup centres (BF16 path)
 // ---------------------------------------------------------------------------
-// The BF16 error bound scales with the squared norms of the (shifted) vectors.
-// Queries are sorted along a Z-order curve of their leading coordinates so that
-// the 128 queries of a workgroup are spatial neighbours; every vector is then
-// shifted by the workgroup's centre, which makes the norms (and the bound)
-// comparable to within-group distances instead of distances from the origin.
-constexpr int kKeyDims = 9;   // dimensions in the Z-order key
-constexpr int kKeyBits = 7;   // bits per dimension (63-bit key)
+// The BF16 error bound scales with the squared norms of the (shifted) vectors,
+// and chunks can only be skipped when they are spatially compact. The points
+// are therefore ordered as the leaves of a balanced kd-tree: the leaves are
+// runs of 128 consecutive positions (the last may be shorter); a node covering
+// leaves [a, b) is split at leaf a + (b - a + 1) / 2 after its points are
+// sorted along the dimension of largest extent within the node. One radix sort
+// per level, keyed by (node, coordinate). Every vector is then shifted by its
+// workgroup's centre, so the norms (and the bound) are comparable to
+// within-group distances instead of distances from the origin.
 constexpr int kWgWaves = 8;   // waves per workgroup in the centred kernel
 constexpr int kWgQueries = kWgWaves * 16;
 constexpr int kChunk = 128;   // database points staged in LDS per step
+static_assert(kChunk == kWgQueries, "database chunk w must hold the queries of workgroup w");
+constexpr int kMaxLevels = 26;    // kd-tree depth bound (2^26 leaves >> INT32_MAX / 128)
+constexpr int kSpreadSlice = 8192;  // positions per block in the extent reduction
 
 __device__ __forceinline__ unsigned int f2ord(float f) {
   const unsigned int u = __float_as_uint(f);
@@ -159,41 +165,85 @@ __device__ __forceinline__ float ord2f(unsigned int o) {
   return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
 }
 
-__global__ void mm_init_kernel(unsigned int* mm) {
-  const int t = threadIdx.x;
-  if (t < 2 * kKeyDims) mm[t] = (t & 1) ? 0u : 0xFFFFFFFFu;
-}
-
-__global__ void minmax_kernel(const double* X, int64_t n, int d, int64_t ld, unsigned int* mm) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int kd = d < kKeyDims ? d : kKeyDims;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * kd; t += stride) {
-    const int64_t i = t / kd;
-    const int c = (int)(t % kd);
-    const unsigned int o = f2ord((float)X[i * ld + c]);
-    atomicMin(&mm[2 * c], o);
-    atomicMax(&mm[2 * c + 1], o);
+// Node (at depth `level`) of leaf `leaf` among nl leaves: the path bits.
+__device__ __forceinline__ uint32_t kd_node(int64_t leaf, int64_t nl, int level) {
+  int64_t a = 0, b = nl;
+  uint32_t id = 0;
+  for (int l = 0; l < level; ++l) {
+    const int64_t m = a + (b - a + 1) / 2;
+    const bool right = leaf >= m;
+    id = (id << 1) | (right ? 1u : 0u);
+    if (right) a = m; else b = m;
   }
+  return id;
 }
 
-__global__ void zkey_kernel(const double* X, int64_t n, int d, int64_t ld, const unsigned int* mm,
-                            uint64_t* keys, uint32_t* idx) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int kd = d < kKeyDims ? d : kKeyDims;
-  uint64_t key = 0;
-  for (int b = kKeyBits - 1; b >= 0; --b) {
-    for (int c = 0; c < kd; ++c) {
-      const float lo = ord2f(mm[2 * c]), hi = ord2f(mm[2 * c + 1]);
-      const float span = hi - lo;
-      float u = span > 0.f ? ((float)X[i * ld + c] - lo) / span : 0.f;
-      u = fminf(fmaxf(u, 0.f), 1.f);
-      const uint32_t q = (uint32_t)fminf(u * (float)(1 << kKeyBits), (float)((1 << kKeyBits) - 1));
-      key = (key << 1) | ((q >> b) & 1u);
+__global__ void kd_mm_init_kernel(unsigned int* mm, int64_t count) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) mm[t] = (t & 1) ? 0u : 0xFFFFFFFFu;
+}
+
+// mm[node][c] = {min, max} (ordered-float bits) over the node's points.
+// Block (node i, slice s) covers positions node_begin[i] + s * kSpreadSlice ...
+__global__ void __launch_bounds__(256) kd_extent_kernel(const double* X, int d, int64_t ld,
+                                                        const uint32_t* order, const int64_t* nodes,
+                                                        int slices, unsigned int* mm) {
+  __shared__ unsigned int lmin[kDim], lmax[kDim];
+  const int64_t i = blockIdx.x / slices;
+  const int s = blockIdx.x % slices;
+  const int64_t b0 = nodes[2 * i], b1 = nodes[2 * i + 1];
+  const int64_t p0 = b0 + (int64_t)s * kSpreadSlice, p1 = min(b1, p0 + kSpreadSlice);
+  if (threadIdx.x < kDim) {
+    lmin[threadIdx.x] = 0xFFFFFFFFu;
+    lmax[threadIdx.x] = 0u;
+  }
+  __syncthreads();
+  if (p0 < p1) {
+    const int c = threadIdx.x & 31;
+    unsigned int mn = 0xFFFFFFFFu, mx = 0u;
+    if (c < d) {
+      for (int64_t p = p0 + (threadIdx.x >> 5); p < p1; p += blockDim.x / 32) {
+        const unsigned int o = f2ord((float)X[(int64_t)order[p] * ld + c]);
+        mn = min(mn, o);
+        mx = max(mx, o);
+      }
+      atomicMin(&lmin[c], mn);
+      atomicMax(&lmax[c], mx);
     }
   }
-  keys[i] = key;
-  idx[i] = (uint32_t)i;
+  __syncthreads();
+  if (p0 < p1 && threadIdx.x < d) {
+    atomicMin(&mm[(i * kDim + threadIdx.x) * 2], lmin[threadIdx.x]);
+    atomicMax(&mm[(i * kDim + threadIdx.x) * 2 + 1], lmax[threadIdx.x]);
+  }
+}
+
+// keys[p] = (node << 32) | ordered bits of the point's coordinate along the
+// node's widest dimension; vals[p] = order[p].
+__global__ void kd_key_kernel(const double* X, int64_t n, int d, int64_t ld, const uint32_t* order,
+                              int64_t nl, int level, const unsigned int* mm, uint64_t* keys,
+                              uint32_t* vals) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t node = kd_node(p / kChunk, nl, level);
+  int best = 0;
+  float ext = -1.f;
+  for (int c = 0; c < d; ++c) {
+    const unsigned int lo = mm[((int64_t)node * kDim + c) * 2], hi = mm[((int64_t)node * kDim + c) * 2 + 1];
+    const float e = hi >= lo ? ord2f(hi) - ord2f(lo) : 0.f;
+    if (e > ext) {
+      ext = e;
+      best = c;
+    }
+  }
+  const uint32_t row = order[p];
+  keys[p] = ((uint64_t)node << 32) | f2ord((float)X[(int64_t)row * ld + best]);
+  vals[p] = row;
+}
+
+__global__ void iota_u32_kernel(uint32_t* v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (uint32_t)i;
 }
 
 // centre[w][c] = mean over the workgroup's queries of X[row][c] (FP64 sum, fixed order)
@@ -212,17 +262,60 @@ __global__ void centre_kernel(const double* X, int64_t n, int d, int64_t ld, con
   centre[w * kDim + c] = (c < d && cnt > 0) ? (float)(s / cnt) : 0.f;
 }
 
+// xs[sp][c] = X[order[sp]][c] (c < d, zero elsewhere): the rows in sorted order
+__global__ void permute_kernel(const double* X, int64_t n, int d, int64_t ld, const uint32_t* order,
+                               int64_t n_rows_pad, double* xs) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t sp = t >> 5;
+  const int c = (int)(t & 31);
+  if (sp >= n_rows_pad) return;
+  xs[t] = (sp < n && c < d) ? X[(int64_t)order[sp] * ld + c] : 0.0;
+}
+
+// radius[w] >= max over the chunk's points of |x - centre[w]| (FP64, rounded up)
+__global__ void __launch_bounds__(kChunk) radius_kernel(const double* xs, int64_t n, const float* centre,
+                                                        double* radius) {
+  __shared__ double red[kChunk / 64];
+  const int64_t w = blockIdx.x;
+  const int64_t sp = w * kChunk + threadIdx.x;
+  double d2 = 0.0;
+  if (sp < n) {
+    for (int c = 0; c < kDim; ++c) {
+      const double v = xs[sp * kDim + c] - (double)centre[w * kDim + c];
+      d2 += v * v;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int i = 0; i < kChunk / 64; ++i) m = fmax(m, red[i]);
+    radius[w] = sqrt(m) * (1.0 + 1e-12) + 1e-300;
+  }
+}
+
 struct CentredParams {
-  const double* X;       // FP64 rows (leading dimension ld)
-  int64_t ld;
+  const double* xs;      // FP64 rows in sorted order, 32 wide
   const float* centre;   // [n_wg][32]
+  const double* radius;  // [n_wg]
   const uint32_t* order; // sorted position -> row
-  int64_t n, n_pad;
+  int64_t n, n_wg;
   int d;
+  int prune;             // skip chunks that cannot hold a candidate
   int32_t* cand;         // [n][kCand] by sorted position
   float* tau;            // [n][4]
   float* qnorm_c;        // [n] |q - centre|^2 by sorted position
+  unsigned long long* visited;  // chunks processed (statistics)
 };
+
+// Visiting order of the database chunks for workgroup w: w, w+1, w-1, w+2,
+// w-2, ... (near chunks first in the spatial sort); -1 past either end.
+__device__ __forceinline__ int64_t visit_chunk(int64_t v, int64_t w, int64_t nwg) {
+  const int64_t half = (v + 1) >> 1;
+  const int64_t k = (v & 1) ? w + half : w - half;
+  return (k >= 0 && k < nwg) ? k : -1;
+}
 
 __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
   h = to_bf16(x);
@@ -232,12 +325,22 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
 }
 
 // Workgroup = 8 waves = 128 consecutive sorted queries sharing one centre.
-// Database chunks of 128 points are loaded once per workgroup, shifted by the
-// centre, given their augmented coordinate |x - c|^2, split into three bf16
-// terms and staged in LDS; every wave then runs its 16 x 16 MFMA tiles on them.
+// Database chunks (128 consecutive sorted points, so chunk w holds the
+// workgroup's own queries) are visited nearest-index first. Before each batch
+// of 64 visits the workgroup takes T = the largest, over its queries, of the
+// smallest stripe threshold (approximate d^2 of the 32nd best); a chunk k is
+// skipped when (|c_w - c_k| - r_w - r_k)^2 > T, a lower bound of the exact d^2
+// between any of its points and any of the queries (triangle inequality on the
+// bounding spheres). Thresholds only shrink, so every skipped point has exact
+// d^2 > the final threshold: the certificate of the finalize step holds for it
+// as for a point that was seen and not kept. Processed chunks are shifted by
+// the centre, given their augmented coordinate |x - c|^2, split into three
+// bf16 terms and staged in LDS; every wave then runs its 16 x 16 MFMA tiles.
 __global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredParams P) {
   __shared__ __bf16 As[3][kChunk][kDim];
   __shared__ float cen[kDim];
+  __shared__ float red[kWgWaves];
+  __shared__ unsigned long long vmask;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int j = lane & 15, g = lane >> 4;
@@ -246,7 +349,6 @@ __global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredPara
   __syncthreads();
   const int64_t qi = wg * kWgQueries + wv * 16 + j;  // sorted position
   const bool qvalid = qi < P.n;
-  const int64_t q = qvalid ? (int64_t)P.order[qi] : 0;
   // query operand: (q - c, -1/2 at dim 30), split in three bf16 terms
   bf16x8 bq_h, bq_m, bq_l;
   float part = 0.f;
@@ -255,7 +357,7 @@ __global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredPara
     const int c = g * 8 + e;
     float v = 0.f;
     // centre in FP64, then round: the rounding error is relative to the shifted value
-    if (c < P.d) v = (float)(P.X[q * P.ld + c] - (double)cen[c]);
+    if (qvalid && c < P.d) v = (float)(P.xs[qi * kDim + c] - (double)cen[c]);
     if (c == kMaxData) v = -0.5f;
     if (c < P.d) part = fmaf(v, v, part);
     __bf16 h, m, l;
@@ -275,64 +377,107 @@ __global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredPara
     L[r] = -INFINITY;
     I[r] = -1;
   }
+  const double rw = P.radius[wg];
+  unsigned long long visited = 0;
   // staging roles: thread t -> point t/4, dims (t%4)*8 .. +7
   const int sp = tid >> 2, sq = tid & 3;
-  for (int64_t c0 = 0; c0 < P.n_pad; c0 += kChunk) {
+  // the first batch is the workgroup's own chunk alone, so that the thresholds
+  // are finite when the other chunks are tested
+  for (int64_t v0 = 0, bs = 1; v0 < 2 * P.n_wg; v0 += bs, bs = 64) {
+    // T: the weakest threshold any query of the workgroup still accepts
+    float tq = INFINITY;
+    {
+      float t = nqc - 2.0f * L[kList - 1];
+      t = fminf(t, __shfl_xor(t, 16, 64));
+      t = fminf(t, __shfl_xor(t, 32, 64));
+      if (!qvalid) t = -INFINITY;
+      for (int o = 1; o < 16; o <<= 1) t = fmaxf(t, __shfl_xor(t, o, 64));
+      tq = t;
+    }
+    __syncthreads();  // previous batch done with vmask and As
+    if (lane == 0) red[wv] = tq;
     __syncthreads();
-    for (int pp = sp; pp < kChunk; pp += (kWgWaves * 64) / 4) {
-      const int64_t p = c0 + pp;
-      float v[8];
-      float ps = 0.f;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int c = sq * 8 + e;
-        v[e] = 0.f;
-        if (p < P.n && c < P.d) v[e] = (float)(P.X[p * P.ld + c] - (double)cen[c]);
-        ps = fmaf(v[e], v[e], ps);
+    if (wv == 0) {
+      float T = red[0];
+      for (int i = 1; i < kWgWaves; ++i) T = fmaxf(T, red[i]);
+      const int64_t k = lane < bs ? visit_chunk(v0 + lane, wg, P.n_wg) : -1;
+      bool take = k >= 0;
+      if (take && P.prune && k != wg && T < INFINITY) {
+        double d2 = 0.0;
+        for (int c = 0; c < kDim; ++c) {
+          const double dc = (double)cen[c] - (double)P.centre[k * kDim + c];
+          d2 += dc * dc;
+        }
+        const double lb = sqrt(d2) * (1.0 - 1e-12) - rw - P.radius[k];
+        take = !(lb > 0.0 && lb * lb > (double)T);
       }
-      ps += __shfl_xor(ps, 1, 64);
-      ps += __shfl_xor(ps, 2, 64);
-      if (sq == 3) v[6] = ps;  // dim 30 = |x - c|^2
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        __bf16 h, m, l;
-        split3(v[e], h, m, l);
-        As[0][pp][sq * 8 + e] = h;
-        As[1][pp][sq * 8 + e] = m;
-        As[2][pp][sq * 8 + e] = l;
-      }
+      const unsigned long long m = __ballot(take);
+      if (lane == 0) vmask = m;
     }
     __syncthreads();
-#pragma unroll 2
-    for (int tt = 0; tt < kChunk / 16; ++tt) {
-      const int row = tt * 16 + j;
-      const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&As[0][row][g * 8]);
-      const bf16x8 am = *reinterpret_cast<const bf16x8*>(&As[1][row][g * 8]);
-      const bf16x8 al = *reinterpret_cast<const bf16x8*>(&As[2][row][g * 8]);
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bq_h, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bq_m, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_l, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bq_h, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_m, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_h, acc, 0, 0, 0);
-      const float th = L[kList - 1];
-      const int64_t pbase = c0 + tt * 16 + g * 4;
+    unsigned long long m = vmask;
+    while (m != 0) {
+      const int bit = __ffsll(m) - 1;
+      m &= m - 1;
+      const int64_t c0 = visit_chunk(v0 + bit, wg, P.n_wg) * kChunk;
+      ++visited;
+      __syncthreads();
+      for (int pp = sp; pp < kChunk; pp += (kWgWaves * 64) / 4) {
+        const int64_t p = c0 + pp;
+        float v[8];
+        float ps = 0.f;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float sv = acc[v];
-        const int64_t p = pbase + v;
-        const bool ins = sv > th && p < P.n;
-        if (__ballot(ins)) {
-          if (ins) list_insert(L, I, sv, (int32_t)p);
+        for (int e = 0; e < 8; ++e) {
+          const int c = sq * 8 + e;
+          v[e] = 0.f;
+          if (p < P.n && c < P.d) v[e] = (float)(P.xs[p * kDim + c] - (double)cen[c]);
+          ps = fmaf(v[e], v[e], ps);
+        }
+        ps += __shfl_xor(ps, 1, 64);
+        ps += __shfl_xor(ps, 2, 64);
+        if (sq == 3) v[6] = ps;  // dim 30 = |x - c|^2
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          __bf16 h, mm, l;
+          split3(v[e], h, mm, l);
+          As[0][pp][sq * 8 + e] = h;
+          As[1][pp][sq * 8 + e] = mm;
+          As[2][pp][sq * 8 + e] = l;
+        }
+      }
+      __syncthreads();
+#pragma unroll 2
+      for (int tt = 0; tt < kChunk / 16; ++tt) {
+        const int row = tt * 16 + j;
+        const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&As[0][row][g * 8]);
+        const bf16x8 am = *reinterpret_cast<const bf16x8*>(&As[1][row][g * 8]);
+        const bf16x8 al = *reinterpret_cast<const bf16x8*>(&As[2][row][g * 8]);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bq_h, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bq_m, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_l, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bq_h, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_m, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_h, acc, 0, 0, 0);
+        const float th = L[kList - 1];
+        const int64_t pbase = c0 + tt * 16 + g * 4;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float sv = acc[u];
+          const int64_t p = pbase + u;
+          const bool ins = sv > th && p < P.n;
+          if (__ballot(ins)) {
+            if (ins) list_insert(L, I, sv, (int32_t)p);
+          }
         }
       }
     }
   }
+  if (tid == 0 && P.visited) atomicAdd(P.visited, visited);
   if (!qvalid) return;
   int32_t* out = P.cand + qi * kCand + g * kList;
 #pragma unroll
-  for (int r = 0; r < kList; ++r) out[r] = I[r];
+  for (int r = 0; r < kList; ++r) out[r] = I[r] >= 0 ? (int32_t)P.order[I[r]] : -1;
   P.tau[qi * 4 + g] = (L[kList - 1] == -INFINITY) ? INFINITY : nqc - 2.0f * L[kList - 1];
   if (g == 0) P.qnorm_c[qi] = nqc;
 }
@@ -863,6 +1008,9 @@ struct Workspace {
   uint64_t *keys, *keys2;
   uint32_t *order0, *order;
   unsigned int* mm;
+  int64_t* kd_nodes;
+  double *xs, *radius;
+  unsigned long long* visited;
   void* sort_tmp;
   size_t sort_bytes;
 };
@@ -901,7 +1049,12 @@ static int64_t layout(int64_t n, Workspace* w, char* base) {
   t.keys2 = (uint64_t*)take(n * 8);
   t.order0 = (uint32_t*)take(n * 4);
   t.order = (uint32_t*)take(n * 4);
-  t.mm = (unsigned int*)take(2 * kKeyDims * 4);
+  const int64_t nl = ceil_div(n, kChunk);
+  t.mm = (unsigned int*)take(nl * kDim * 2 * 4);
+  t.kd_nodes = (int64_t*)take((4 * nl + 64) * 2 * 8);
+  t.xs = (double*)take(ceil_div(n, kChunk) * kChunk * kDim * 8);
+  t.radius = (double*)take(ceil_div(n, kChunk) * 8);
+  t.visited = (unsigned long long*)take(8);
   size_t sb = 0;
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sb, (const uint64_t*)nullptr, (uint64_t*)nullptr,
                                            (const uint32_t*)nullptr, (uint32_t*)nullptr,
@@ -929,7 +1082,8 @@ int64_t snapgpu_knn_workspace_bytes(int64_t n, int32_t d, int32_t k) {
 
 int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, int32_t mode,
                 int32_t* d_nbr, double* d_dist, void* d_workspace, int64_t workspace_bytes,
-                int32_t flags, int64_t* n_fallback, int64_t* n_exact, void* stream) {
+                int32_t flags, int64_t* n_fallback, int64_t* n_exact, int64_t* n_chunk_visits,
+                void* stream) {
   clear_error();
   SG_ARG(n >= 2, "need at least 2 points");
   SG_ARG(n < INT32_MAX, "too many points");
@@ -956,16 +1110,50 @@ int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, 
                d_nbr, d_dist, w.fail_rows, w.fail_count};
   if (mode == kModeBF16) {
     // spatial order of the queries, per-workgroup centres, centred LDS-staged MFMA tiles
-    mm_init_kernel<<<1, 64, 0, st>>>(w.mm);
-    minmax_kernel<<<grid_knn(n * std::min(d, kKeyDims)), 256, 0, st>>>(d_X, n, d, ld, w.mm);
-    zkey_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(d_X, n, d, ld, w.mm, w.keys, w.order0);
-    SG_LAUNCH_CHECK();
-    size_t sb = w.sort_bytes;
-    SG_HIP(hipcub::DeviceRadixSort::SortPairs(w.sort_tmp, sb, w.keys, w.keys2, w.order0, w.order,
-                                              (int)n, 0, kKeyDims * kKeyBits, st));
+    // kd-tree order: leaves of 128 consecutive positions, split along the widest dimension
+    const int64_t nl = ceil_div(n, kChunk);
+    int levels = 0;
+    while (((int64_t)1 << levels) < nl) ++levels;
+    SG_ARG(levels <= kMaxLevels, "too many points for the kd-tree order");
+    std::vector<int64_t> tab;
+    std::vector<int64_t> level_off(levels + 1, 0);
+    for (int l = 0; l < levels; ++l) {
+      level_off[l] = (int64_t)tab.size() / 2;
+      for (int64_t id = 0; id < ((int64_t)1 << l); ++id) {
+        int64_t a = 0, b = nl;
+        for (int q = l - 1; q >= 0; --q) {
+          const int64_t m = a + (b - a + 1) / 2;
+          if ((id >> q) & 1) a = m; else b = m;
+        }
+        tab.push_back(std::min(a * kChunk, n));
+        tab.push_back(std::min(b * kChunk, n));
+      }
+    }
+    if (!tab.empty())
+      SG_HIP(hipMemcpyAsync(w.kd_nodes, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, st));
+    iota_u32_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(w.order, n);
+    for (int l = 0; l < levels; ++l) {
+      const int64_t nodes = (int64_t)1 << l;
+      const int64_t max_size = ceil_div(nl, nodes) * kChunk;
+      const int slices = (int)std::max<int64_t>(1, ceil_div(max_size, kSpreadSlice));
+      kd_mm_init_kernel<<<(unsigned)ceil_div(nodes * kDim * 2, 256), 256, 0, st>>>(w.mm, nodes * kDim * 2);
+      kd_extent_kernel<<<(unsigned)(nodes * slices), 256, 0, st>>>(d_X, d, ld, w.order,
+                                                                  w.kd_nodes + 2 * level_off[l], slices, w.mm);
+      kd_key_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(d_X, n, d, ld, w.order, nl, l, w.mm, w.keys,
+                                                               w.order0);
+      SG_LAUNCH_CHECK();
+      size_t sb = w.sort_bytes;
+      SG_HIP(hipcub::DeviceRadixSort::SortPairs(w.sort_tmp, sb, w.keys, w.keys2, w.order0, w.order, (int)n, 0,
+                                                32 + l, st));
+    }
     const int64_t n_wg = ceil_div(n, kWgQueries);
     centre_kernel<<<(unsigned)n_wg, 32, 0, st>>>(d_X, n, d, ld, w.order, w.centre);
-    CentredParams cc{d_X, ld, w.centre, w.order, n, n_pad, d, w.cand, w.tau, w.qnorm_c};
+    permute_kernel<<<(unsigned)ceil_div(n_wg * kChunk * kDim, 256), 256, 0, st>>>(d_X, n, d, ld, w.order,
+                                                                                n_wg * kChunk, w.xs);
+    radius_kernel<<<(unsigned)n_wg, kChunk, 0, st>>>(w.xs, n, w.centre, w.radius);
+    SG_HIP(hipMemsetAsync(w.visited, 0, 8, st));
+    CentredParams cc{w.xs, w.centre, w.radius, w.order, n, n_wg, d, (flags & 2) ? 0 : 1,
+                     w.cand, w.tau, w.qnorm_c, w.visited};
     cand_centred_kernel<<<(unsigned)n_wg, kWgWaves * 64, 0, st>>>(cc);
     SG_LAUNCH_CHECK();
     fp.rows = reinterpret_cast<const int32_t*>(w.order);
@@ -1004,6 +1192,14 @@ int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, 
     fallback_kernel<<<(unsigned)std::min<int64_t>(nfail, 65535), 64, 0, st>>>(fp, w.fail_rows,
                                                                               w.fail_count);
     SG_LAUNCH_CHECK();
+  }
+  if (n_chunk_visits) {
+    unsigned long long nv = 0;
+    if (mode == kModeBF16) {
+      SG_HIP(hipMemcpyAsync(&nv, w.visited, 8, hipMemcpyDeviceToHost, st));
+      SG_HIP(hipStreamSynchronize(st));
+    }
+    *n_chunk_visits = (int64_t)nv;
   }
   if (n_fallback) *n_fallback = (int64_t)nfail;
   if (n_exact) *n_exact = (int64_t)(flags & 1 ? nfail : nfail2);

@@ -21,22 +21,10 @@
 // bitwise reproducible.
 #include <hipcub/hipcub.hpp>
 
-#include "common.h"
+#include "spectral_common.h"
 
 namespace snapgpu {
 namespace spectral {
-
-constexpr int kB = 32;  // dense block width (columns of V, Z, Y)
-
-// Stored values are uint32 counts (tile matrices) or FP32 bit patterns.
-template <bool FV>
-__device__ __forceinline__ double val64(const uint32_t* v, int64_t e) {
-  return FV ? (double)__uint_as_float(v[e]) : (double)v[e];
-}
-template <bool FV>
-__device__ __forceinline__ float val32(const uint32_t* v, int64_t e) {
-  return FV ? __uint_as_float(v[e]) : (float)v[e];
-}
 
 // ---------------------------------------------------------------------------
 // Column selection of a CSR count matrix (feature mask -> new column ids)
@@ -312,166 +300,6 @@ __global__ void __launch_bounds__(256) spmm_kernel(const int64_t* __restrict__ p
   }
 }
 
-// ---------------------------------------------------------------------------
-// Panel-segmented SpMM: the dense operand is processed in panels of kPanel rows
-// (kPanel * BW * 4 bytes: 2 MiB at BW = 16), one launch per panel, so every
-// gathered row is an L2 hit. seg[row][p] (uint32, P+1 per row) is the offset of
-// the row's first entry whose column (CSR) / row (CSC) lies in panel p: both
-// index arrays are sorted within a row, so the segments are contiguous and no
-// data is duplicated. Partial sums are accumulated in FP32 in panel order.
-// ---------------------------------------------------------------------------
-constexpr int64_t kPanel = 32768;
-
-__global__ void __launch_bounds__(256) segment_kernel(const int64_t* __restrict__ ptr,
-                                                      const int32_t* __restrict__ idx, int64_t nrows,
-                                                      int P, uint32_t* __restrict__ seg) {
-  const int lane = threadIdx.x & 63;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; row < nrows; row += nwaves) {
-    const int64_t e0 = ptr[row], len = ptr[row + 1] - e0;
-    for (int pb = lane; pb <= P; pb += 64) {
-      const int64_t bound = (int64_t)pb * kPanel;
-      int64_t lo = 0, hi = len;  // first entry with idx >= bound
-      while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if ((int64_t)idx[e0 + mid] < bound) lo = mid + 1; else hi = mid;
-      }
-      seg[row * (P + 1) + pb] = (uint32_t)lo;
-    }
-  }
-}
-
-// PH bit 0: first panel (no accumulator read); bit 1: last panel (epilogue).
-template <int BW, int EPI, bool FV, int PH>
-__global__ void __launch_bounds__(256) spmm_panel_kernel(
-    const int64_t* __restrict__ ptr, const int32_t* __restrict__ col, const uint32_t* __restrict__ cnt,
-    int64_t nrows, const uint32_t* __restrict__ seg, int P, int p, const float* __restrict__ dense,
-    const float* __restrict__ rowscale, float* __restrict__ acc32, const double* __restrict__ V,
-    int64_t ldv, const double* __restrict__ dinv, double* __restrict__ Y, int64_t ldy, int b) {
-  constexpr int G = 64 / BW;
-  const int lane = threadIdx.x & 63;
-  const int g = lane / BW;
-  const int c = lane % BW;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; row < nrows; row += nwaves) {
-    const int64_t base_e = ptr[row];
-    const int64_t e0 = base_e + seg[row * (P + 1) + p];
-    const int64_t e1 = base_e + seg[row * (P + 1) + p + 1];
-    float acc = 0.f;
-    if (e1 > e0) {
-      int32_t j = 0;
-      float v = 0.f;
-      if (e0 + lane < e1) {
-        j = __builtin_nontemporal_load(col + e0 + lane);
-        v = val32<FV>(cnt, e0 + lane);
-      }
-      for (int64_t base = e0; base < e1; base += 64) {
-        int32_t jn = 0;
-        float vn = 0.f;
-        const int64_t en = base + 64 + lane;
-        if (en < e1) {
-          jn = __builtin_nontemporal_load(col + en);
-          vn = val32<FV>(cnt, en);
-        }
-        const int nb = (int)min((int64_t)64, e1 - base);
-        const int steps = (nb + G - 1) / G;
-        int s = 0;
-        for (; s + 8 <= steps; s += 8) {
-          float z[8], vv[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int t = (s + u) * G + g;
-            const int32_t jt = __shfl(j, t, 64);
-            const float vt = __shfl(v, t, 64);
-            vv[u] = t < nb ? vt : 0.f;
-            z[u] = dense[(int64_t)(t < nb ? jt : 0) * BW + c];
-          }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) acc = fmaf(vv[u], z[u], acc);
-        }
-        for (; s < steps; ++s) {
-          const int t = s * G + g;
-          const int32_t jt = __shfl(j, t, 64);
-          const float vt = __shfl(v, t, 64);
-          if (t < nb) acc = fmaf(vt, dense[(int64_t)jt * BW + c], acc);
-        }
-        j = jn;
-        v = vn;
-      }
-#pragma unroll
-      for (int o = BW; o < 64; o <<= 1) acc += __shfl_xor(acc, o, 64);
-    }
-    if (g != 0) continue;
-    float a = acc;
-    if (!(PH & 1)) a += acc32[row * BW + c];
-    if (PH & 2) {
-      if (EPI == 0) {
-        acc32[row * BW + c] = a * rowscale[row];
-      } else if (c < b) {
-        Y[row * ldy + c] = (double)(a * rowscale[row]) - dinv[row] * V[row * ldv + c];
-      }
-    } else {
-      acc32[row * BW + c] = a;
-    }
-  }
-}
-
-template <int BW, int EPI, bool FV>
-static void launch_panels(const int64_t* ptr, const int32_t* col, const uint32_t* cnt, int64_t nrows,
-                          const uint32_t* seg, int P, const float* dense, const float* rowscale,
-                          float* acc32, const double* V, int64_t ldv, const double* dinv, double* Y,
-                          int64_t ldy, int b, hipStream_t st) {
-  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nrows, 4), 16384));
-  for (int p = 0; p < P; ++p) {
-    const int ph = (p == 0 ? 1 : 0) | (p == P - 1 ? 2 : 0);
-#define SG_PANEL(PHV)                                                                              \
-  spmm_panel_kernel<BW, EPI, FV, PHV><<<grid, 256, 0, st>>>(ptr, col, cnt, nrows, seg, P, p, dense, \
-                                                             rowscale, acc32, V, ldv, dinv, Y, ldy, b)
-    switch (ph) {
-      case 0: SG_PANEL(0); break;
-      case 1: SG_PANEL(1); break;
-      case 2: SG_PANEL(2); break;
-      default: SG_PANEL(3); break;
-    }
-#undef SG_PANEL
-  }
-}
-
-template <int BW, bool FV>
-static void launch_apply_panels(const int64_t* indptr, const int32_t* idx, const uint32_t* cnt, int64_t n,
-                                const int64_t* colptr, const int32_t* rowidx, const uint32_t* ccnt,
-                                int64_t m, const uint32_t* seg_csr, int Pc, const uint32_t* seg_csc,
-                                int Pr, const float* r32, const float* w2, const double* dinv,
-                                const double* V, int64_t ldv, int b, double* Y, int64_t ldy, float* Vs,
-                                float* Z, hipStream_t st);
-
-// Vs[i][c] = r_i * V[i][c] as FP32 (row stride BW), columns >= b zero-filled.
-template <int BW>
-__global__ void scale_in_kernel(const double* V, int64_t ldv, int64_t n, int b, const float* r,
-                                float* Vs) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t i = t / BW;
-  const int c = (int)(t % BW);
-  if (i >= n) return;
-  Vs[i * BW + c] = c < b ? (float)((double)r[i] * V[i * ldv + c]) : 0.f;
-}
-
-template <int BW, bool FV>
-static void launch_apply_panels(const int64_t* indptr, const int32_t* idx, const uint32_t* cnt, int64_t n,
-                                const int64_t* colptr, const int32_t* rowidx, const uint32_t* ccnt,
-                                int64_t m, const uint32_t* seg_csr, int Pc, const uint32_t* seg_csc,
-                                int Pr, const float* r32, const float* w2, const double* dinv,
-                                const double* V, int64_t ldv, int b, double* Y, int64_t ldy, float* Vs,
-                                float* Z, hipStream_t st) {
-  scale_in_kernel<BW><<<(unsigned)ceil_div(n * BW, 256), 256, 0, st>>>(V, ldv, n, b, r32, Vs);
-  // Z = w^2 o (X^T Vs): CSC rows are features, panels over cells (rows of Vs)
-  launch_panels<BW, 0, FV>(colptr, rowidx, ccnt, m, seg_csc, Pr, Vs, w2, Z, nullptr, 0, nullptr,
-                           nullptr, 0, b, st);
-  // Y = r o (X Z) - dinv o V: panels over features (rows of Z); Vs is reused as
-  // the FP32 accumulator
-  launch_panels<BW, 1, FV>(indptr, idx, cnt, n, seg_csr, Pc, Z, r32, Vs, V, ldv, dinv, Y, ldy, b, st);
-}
-
 template <int BW, bool FV>
 static void launch_apply(const int64_t* indptr, const int32_t* idx, const uint32_t* cnt, int64_t n,
                          const int64_t* colptr, const int32_t* rowidx, const uint32_t* ccnt,
@@ -534,29 +362,54 @@ __global__ void reduce_partials_kernel(const double* partial, int64_t nchunks, i
   C[i] = s;
 }
 
-// C[i][c] = beta*C[i][c] + alpha * sum_a A[i][a] B[a][c]; block = 8 rows x 32 columns.
-__global__ void __launch_bounds__(256) gemm_nn_kernel(const double* A, int64_t lda, const double* B,
-                                                      int64_t n, int ka, int kb, double alpha,
-                                                      double beta, double* C, int64_t ldc) {
+// C[i][c] = beta*C[i][c] + alpha * sum_a A[i][a] B[a][c]. A block owns 64 rows:
+// 32-column slices of A are staged in LDS with coalesced loads (a wave reads two
+// 256-byte row segments per instruction); thread (row t/4, columns t%4 + 4j)
+// keeps up to 8 accumulators. The sum runs over a in ascending order.
+constexpr int kNnRows = 64;
+__global__ void __launch_bounds__(256) gemm_nn_kernel(const double* __restrict__ A, int64_t lda,
+                                                      const double* __restrict__ B, int64_t n, int ka,
+                                                      int kb, double alpha, double beta,
+                                                      double* __restrict__ C, int64_t ldc) {
+  __shared__ double As[kNnRows][33];
   __shared__ double Bs[32][33];
-  const int c = threadIdx.x & 31;
-  const int64_t i = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);
-  double acc = 0.0;
+  const int t = threadIdx.x;
+  const int r = t >> 2, cg = t & 3;
+  const int64_t row0 = (int64_t)blockIdx.x * kNnRows;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int a0 = 0; a0 < ka; a0 += 32) {
-    for (int q = threadIdx.x; q < 32 * 32; q += 256) {
-      const int ra = q >> 5, cc = q & 31;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int e = k * 256 + t;
+      const int rr = e >> 5, cc = e & 31;
+      const int64_t i = row0 + rr;
+      As[rr][cc] = (i < n && a0 + cc < ka) ? A[i * lda + a0 + cc] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = k * 256 + t;
+      const int ra = e >> 5, cc = e & 31;
       Bs[ra][cc] = (a0 + ra < ka && cc < kb) ? B[(int64_t)(a0 + ra) * kb + cc] : 0.0;
     }
     __syncthreads();
-    if (i < n) {
-      const int lim = min(32, ka - a0);
-      for (int ra = 0; ra < lim; ++ra) acc = fma(A[i * lda + a0 + ra], Bs[ra][c], acc);
+    const int lim = min(32, ka - a0);
+    for (int ra = 0; ra < lim; ++ra) {
+      const double av = As[r][ra];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[j] = fma(av, Bs[ra][cg + 4 * j], acc[j]);
     }
     __syncthreads();
   }
-  if (i < n && c < kb) {
-    const double prev = beta == 0.0 ? 0.0 : beta * C[i * ldc + c];
-    C[i * ldc + c] = prev + alpha * acc;
+  const int64_t i = row0 + r;
+  if (i < n) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int c = cg + 4 * j;
+      if (c < kb) {
+        const double prev = beta == 0.0 ? 0.0 : beta * C[i * ldc + c];
+        C[i * ldc + c] = prev + alpha * acc[j];
+      }
+    }
   }
 }
 
@@ -980,60 +833,6 @@ int snapgpu_spectral_vjp_w(const int64_t* d_indptr, const int32_t* d_idx, const 
   return SNAPGPU_OK;
 }
 
-int snapgpu_spectral_segments(const int64_t* d_ptr, const int32_t* d_idx, int64_t nrows, int64_t ncols_dense,
-                              uint32_t* d_seg, void* stream) {
-  clear_error();
-  SG_ARG(nrows >= 0 && ncols_dense >= 1, "bad shape");
-  if (nrows == 0) return SNAPGPU_OK;
-  const int P = (int)ceil_div(ncols_dense, kPanel);
-  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nrows, 4), 16384));
-  segment_kernel<<<grid, 256, 0, as_stream(stream)>>>(d_ptr, d_idx, nrows, P, d_seg);
-  SG_LAUNCH_CHECK();
-  return SNAPGPU_OK;
-}
-
-int32_t snapgpu_spectral_panel_count(int64_t dense_rows) {
-  return (int32_t)std::max<int64_t>(1, ceil_div(dense_rows, kPanel));
-}
-
-int snapgpu_spectral_apply_panels(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_cnt,
-                                  int64_t n, const int64_t* d_colptr, const int32_t* d_rowidx,
-                                  const uint32_t* d_ccnt, int64_t m, const uint32_t* d_seg_csr,
-                                  const uint32_t* d_seg_csc, const float* d_r32, const float* d_w2,
-                                  const double* d_dinv, const double* d_V, int64_t ldv, int32_t b,
-                                  double* d_Y, int64_t ldy, int32_t value_type, void* d_ws,
-                                  int64_t ws_bytes, void* stream) {
-  clear_error();
-  SG_ARG(b >= 1 && b <= kB, "block width must be in [1, 32]");
-  SG_ARG(ldv >= b && ldy >= b, "leading dimension < b");
-  SG_ARG(value_type == 0 || value_type == 1, "value_type must be 0 (uint32) or 1 (float32)");
-  SG_ARG(ws_bytes >= snapgpu_spectral_apply_workspace_bytes(n, m), "workspace too small");
-  SG_ARG(d_seg_csr && d_seg_csc, "null segment table");
-  hipStream_t st = as_stream(stream);
-  float* Vs = (float*)(((uintptr_t)d_ws + 255) & ~(uintptr_t)255);
-  float* Z = Vs + n * kB;
-  const int Pc = (int)std::max<int64_t>(1, ceil_div(m, kPanel));
-  const int Pr = (int)std::max<int64_t>(1, ceil_div(n, kPanel));
-  const bool fv = value_type == 1;
-#define SG_APPLY_P(BW)                                                                                  \
-  if (fv) launch_apply_panels<BW, true>(d_indptr, d_idx, d_cnt, n, d_colptr, d_rowidx, d_ccnt, m,      \
-                                        d_seg_csr, Pc, d_seg_csc, Pr, d_r32, d_w2, d_dinv, d_V, ldv, b, \
-                                        d_Y, ldy, Vs, Z, st);                                           \
-  else launch_apply_panels<BW, false>(d_indptr, d_idx, d_cnt, n, d_colptr, d_rowidx, d_ccnt, m,        \
-                                      d_seg_csr, Pc, d_seg_csc, Pr, d_r32, d_w2, d_dinv, d_V, ldv, b,   \
-                                      d_Y, ldy, Vs, Z, st);
-  if (b <= 8) {
-    SG_APPLY_P(8)
-  } else if (b <= 16) {
-    SG_APPLY_P(16)
-  } else {
-    SG_APPLY_P(32)
-  }
-#undef SG_APPLY_P
-  SG_LAUNCH_CHECK();
-  return SNAPGPU_OK;
-}
-
 int64_t snapgpu_gemm_tn_workspace_bytes(int64_t n, int32_t ka, int32_t kb) {
   const int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(512, ceil_div(n, 4096)));
   return chunks * (int64_t)ka * kb * 8 + 256;
@@ -1066,8 +865,8 @@ int snapgpu_gemm_nn_f64(const double* d_A, int64_t lda, const double* d_B, int64
   clear_error();
   SG_ARG(ka >= 1 && kb >= 1 && kb <= 32, "need 1 <= kb <= 32");
   if (n == 0) return SNAPGPU_OK;
-  gemm_nn_kernel<<<(unsigned)ceil_div(n, 8), 256, 0, as_stream(stream)>>>(d_A, lda, d_B, n, ka, kb,
-                                                                          alpha, beta, d_C, ldc);
+  gemm_nn_kernel<<<(unsigned)ceil_div(n, kNnRows), 256, 0, as_stream(stream)>>>(d_A, lda, d_B, n, ka,
+                                                                                kb, alpha, beta, d_C, ldc);
   SG_LAUNCH_CHECK();
   return SNAPGPU_OK;
 }

@@ -18,10 +18,11 @@ _MODES = {"fp32": 0, "bf16": 1}
 
 
 def nearest_neighbour_graph(data, k: int, precision: str = "bf16", force_fallback: bool = False,
-                            return_stats: bool = False):
+                            return_stats: bool = False, prune: bool = True):
     """Exact kNN graph of the rows of `data` (n x d, d <= 30): scipy CSR n x n,
     float64 Euclidean distances, k entries per row sorted by neighbour index,
-    self excluded by index. `data` may be a numpy array or a CUDA torch tensor."""
+    self excluded by index. `data` may be a numpy array or a CUDA torch tensor.
+    prune=False makes the bf16 path visit every database chunk (same result)."""
     torch = _lib.require_gpu()
     L = _lib.lib()
     if precision not in _MODES:
@@ -37,19 +38,24 @@ def nearest_neighbour_graph(data, k: int, precision: str = "bf16", force_fallbac
     if kk < 1:
         # the kd-tree returns no neighbours for a single point
         out = csr_matrix((n, n), dtype=np.float64)
-        return (out, {"fallback_rows": 0, "exact_scan_rows": 0}) if return_stats else out
+        return (out, {"fallback_rows": 0, "exact_scan_rows": 0, "chunk_visits": 0,
+                      "chunk_pairs": 0}) if return_stats else out
     nbr = torch.empty((n, kk), dtype=torch.int32, device=X.device)
     dist = torch.empty((n, kk), dtype=torch.float64, device=X.device)
     ws_b = int(L.snapgpu_knn_workspace_bytes(n, d, kk))
     ws = torch.empty(ws_b, dtype=torch.uint8, device=X.device)
     nfb = C.c_int64(0)
     nex = C.c_int64(0)
+    nvis = C.c_int64(0)
+    flags = (1 if force_fallback else 0) | (0 if prune else 2)
     _lib.check(L.snapgpu_knn(X.data_ptr(), n, d, d, kk, _MODES[precision], nbr.data_ptr(),
-                             dist.data_ptr(), ws.data_ptr(), ws_b, 1 if force_fallback else 0,
-                             C.byref(nfb), C.byref(nex), _lib.stream_ptr()))
+                             dist.data_ptr(), ws.data_ptr(), ws_b, flags, C.byref(nfb), C.byref(nex),
+                             C.byref(nvis), _lib.stream_ptr()))
     indptr = np.arange(0, n * kk + 1, kk, dtype=np.int64)
     out = csr_matrix((dist.cpu().numpy().ravel(), nbr.cpu().numpy().ravel(), indptr), shape=(n, n))
-    stats = {"fallback_rows": int(nfb.value), "exact_scan_rows": int(nex.value)}
+    nwg = -(-n // 128)
+    stats = {"fallback_rows": int(nfb.value), "exact_scan_rows": int(nex.value),
+             "chunk_visits": int(nvis.value), "chunk_pairs": nwg * nwg if precision == "bf16" else 0}
     return (out, stats) if return_stats else out
 
 

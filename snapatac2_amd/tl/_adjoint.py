@@ -89,19 +89,20 @@ def normalized_cosine(V, w, op: SpectralOperator):
     return _FN.apply(V, w, op)
 
 
-def spectral_gap(op: SpectralOperator, n_components: int = 30, seed: int = 0):
+def spectral_gap(op: SpectralOperator, n_components: int = 30, seed: int = 0, v0=None):
     """(gap, evals, U): gap = |lambda_k| - |lambda_{k+1}| between the n_components-th
-    and the next eigenvalue in the largest-magnitude order used by spectral()."""
-    evals, U, _ = eigsh_lm(op, n_components + 1, seed=seed)
+    and the next eigenvalue in the largest-magnitude order used by spectral().
+    v0: eigenvectors of a nearby operator to start the eigensolver from."""
+    evals, U, _ = eigsh_lm(op, n_components + 1, seed=seed, v0=v0)
     k = n_components
     return float(abs(evals[k - 1]) - abs(evals[k])), evals, U
 
 
-def spectral_gap_grad(op: SpectralOperator, n_components: int = 30, seed: int = 0):
+def spectral_gap_grad(op: SpectralOperator, n_components: int = 30, seed: int = 0, v0=None):
     """(gap, dgap/dw, evals, U) by Hellmann-Feynman: d lambda_i = u_i^T dA u_i, i.e. the adjoint
     with V = [u_k, u_{k+1}] and G = [sign(lambda_k) u_k, -sign(lambda_{k+1}) u_{k+1}]."""
     torch = op.torch
-    gap, evals, U = spectral_gap(op, n_components, seed)
+    gap, evals, U = spectral_gap(op, n_components, seed, v0)
     k = n_components
     V = U[:, k - 1:k + 1].contiguous()
     sg = torch.tensor([np.sign(evals[k - 1]) or 1.0, -(np.sign(evals[k]) or 1.0)],
@@ -120,7 +121,8 @@ def tune_feature_weights(op: SpectralOperator, n_components: int = 30, steps: in
     weights, the (gap, step_size) history of the accepted iterates, and the
     n_components + 1 eigenpairs of largest magnitude for the returned weights,
     so a caller can embed with them without another eigensolve. `op` is left
-    prepared for the returned weights."""
+    prepared for the returned weights. Each trial eigensolve starts from the
+    eigenvectors of the current weights."""
     torch = op.torch
     w = op.w.clone()
     gap, g, evals, U = spectral_gap_grad(op, n_components, seed)
@@ -137,10 +139,10 @@ def tune_feature_weights(op: SpectralOperator, n_components: int = 30, steps: in
         try:
             op.set_weights(trial)
             if last:  # no further step: the gradient at the trial point is not needed
-                new_gap, new_evals, new_U = spectral_gap(op, n_components, seed)
+                new_gap, new_evals, new_U = spectral_gap(op, n_components, seed, v0=U)
                 new_g = None
             else:
-                new_gap, new_g, new_evals, new_U = spectral_gap_grad(op, n_components, seed)
+                new_gap, new_g, new_evals, new_U = spectral_gap_grad(op, n_components, seed, v0=U)
         except _lib.PanicException:
             new_gap = -np.inf
         if new_gap > gap:

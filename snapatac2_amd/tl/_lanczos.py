@@ -101,9 +101,13 @@ def dense_eigsh(op, k: int):
 
 
 def eigsh_lm(op, k: int, block: int = 16, max_basis: int | None = None, tol: float = 1e-9,
-             seed: int = 0, max_restarts: int = 400, dense_below: int = 1024, check_every: int = 4):
+             seed: int = 0, max_restarts: int = 400, dense_below: int = 1024, check_every: int = 4,
+             v0=None):
     """k largest-magnitude eigenpairs of the symmetric operator `op`.
 
+    v0: optional n x m device tensor (e.g. eigenvectors of a nearby operator).
+    The start block is then v0 times a seeded random m x block matrix, so the
+    first block steps span v0's columns; otherwise the start block is random.
     Returns (evals (host), evecs (device tensor, n x k), stats) in LM order.
     """
     torch = op.torch
@@ -165,6 +169,10 @@ def eigsh_lm(op, k: int, block: int = 16, max_basis: int | None = None, tol: flo
 
     # start block
     la.random(W, rng_seed)
+    if v0 is not None:
+        m0 = int(v0.shape[1])
+        M = np.random.default_rng(rng_seed).standard_normal((m0, b))
+        la.nn(v0.to(torch.float64).contiguous(), M, W, 1.0, 1e-3)
     R0 = cholqr2(W, V[:, :b])
     if R0 is None:
         raise RuntimeError("degenerate start block")

@@ -109,23 +109,54 @@ class SpectralOperator:
         self.w2 = torch.empty(m, dtype=torch.float32, device=dev)
         self._status = torch.zeros(8, dtype=torch.int32, device=dev)
         self._prepare()
-        # panel segment tables (dense operands larger than one L2-sized panel)
-        pc = int(L.snapgpu_spectral_panel_count(m))
-        pr = int(L.snapgpu_spectral_panel_count(n))
+        # panel layouts (dense operands larger than one L2-sized panel)
+        pc = int(L.snapgpu_panel_count(m))
+        pr = int(L.snapgpu_panel_count(n))
         self.use_panels = (pc > 1 or pr > 1) if use_panels is None else bool(use_panels)
+        self._panel_keep = []
         if self.use_panels:
-            self.seg_csr = torch.empty(n * (pc + 1), dtype=torch.int32, device=dev)
-            self.seg_csc = torch.empty(m * (pr + 1), dtype=torch.int32, device=dev)
-            _lib.check(L.snapgpu_spectral_segments(X.indptr.data_ptr(), X.indices.data_ptr(), n, m,
-                                                   self.seg_csr.data_ptr(), st))
-            _lib.check(L.snapgpu_spectral_segments(self.colptr.data_ptr(), self.rowidx.data_ptr(),
-                                                   m, n, self.seg_csc.data_ptr(), st))
+            self.panel_csr = self._build_panel(X.indptr, X.indices, X.data, n, m, X.nnz)
+            self.panel_csc = self._build_panel(self.colptr, self.rowidx, self.cval, m, n, X.nnz)
+            self._pws_b = int(L.snapgpu_spectral_apply_panel_workspace_bytes(
+                n, m, self.panel_csr.n_pairs, self.panel_csc.n_pairs))
+            self._pws = torch.empty(self._pws_b, dtype=torch.uint8, device=dev)
         self._ws_b = int(L.snapgpu_spectral_apply_workspace_bytes(n, m))
         self._ws = torch.empty(self._ws_b, dtype=torch.uint8, device=dev)
         self.n_applies = 0
         self.n_columns = 0
         self.events = None  # optional list collecting (start, end) events per application
         self._vjp_ws = None
+
+    def _build_panel(self, ptr, idx, val, nrows: int, dense_rows: int, nnz: int):
+        """Panel-major copy of one storage direction (see snapgpu_panel_build)."""
+        torch = self.torch
+        L = _lib.lib()
+        dev = ptr.device
+        st = _lib.stream_ptr()
+        ws_b = int(L.snapgpu_panel_build_workspace_bytes(nrows, dense_rows, nnz))
+        ws = torch.empty(ws_b, dtype=torch.uint8, device=dev)
+        K, NC, packed = C.c_int64(), C.c_int64(), C.c_int32()
+        _lib.check(L.snapgpu_panel_sizes(ptr.data_ptr(), idx.data_ptr(), val.data_ptr(), nrows, dense_rows,
+                                         nnz, self.value_type, ws.data_ptr(), ws_b, C.byref(K),
+                                         C.byref(NC), C.byref(packed), st))
+        i32 = dict(dtype=torch.int32, device=dev)
+        ent = torch.empty(max(nnz, 1), **i32)
+        val_out = None if packed.value else torch.empty(max(nnz, 1), **i32)
+        cstart = torch.empty(NC.value + 1, dtype=torch.int64, device=dev)
+        cpair = torch.empty(NC.value + 1, **i32)
+        cpanel = torch.empty(max(NC.value, 1), **i32)
+        rp_ptr = torch.empty(nrows + 1, dtype=torch.int64, device=dev)
+        rp_ids = torch.empty(max(K.value, 1), **i32)
+        _lib.check(L.snapgpu_panel_build(
+            ptr.data_ptr(), idx.data_ptr(), val.data_ptr(), nrows, dense_rows, nnz, packed.value,
+            ws.data_ptr(), ws_b, ent.data_ptr(), val_out.data_ptr() if val_out is not None else None,
+            cstart.data_ptr(), cpair.data_ptr(), cpanel.data_ptr(), NC.value, rp_ptr.data_ptr(),
+            rp_ids.data_ptr(), st))
+        self._panel_keep.append((ent, val_out, cstart, cpair, cpanel, rp_ptr, rp_ids))
+        return _lib.PanelLayout(ent.data_ptr(), val_out.data_ptr() if val_out is not None else None,
+                                cstart.data_ptr(), cpair.data_ptr(), cpanel.data_ptr(), NC.value,
+                                rp_ptr.data_ptr(), rp_ids.data_ptr(), K.value, nrows, dense_rows,
+                                packed.value, self.value_type)
 
     def _load_weights(self, w) -> None:
         torch = self.torch
@@ -195,13 +226,10 @@ class SpectralOperator:
             ev = (self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True))
             ev[0].record()
         if self.use_panels:
-            _lib.check(L.snapgpu_spectral_apply_panels(
-                X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), self.n,
-                self.colptr.data_ptr(), self.rowidx.data_ptr(), self.cval.data_ptr(), self.m,
-                self.seg_csr.data_ptr(), self.seg_csc.data_ptr(), self.r32.data_ptr(),
-                self.w2.data_ptr(), self.dinv.data_ptr(), V.data_ptr(), V.stride(0), b,
-                Y.data_ptr(), Y.stride(0), self.value_type, self._ws.data_ptr(), self._ws_b,
-                _lib.stream_ptr()))
+            _lib.check(L.snapgpu_spectral_apply_panel(
+                C.byref(self.panel_csr), C.byref(self.panel_csc), self.r32.data_ptr(),
+                self.w2.data_ptr(), self.dinv.data_ptr(), V.data_ptr(), V.stride(0), b, Y.data_ptr(),
+                Y.stride(0), self._pws.data_ptr(), self._pws_b, _lib.stream_ptr()))
         else:
             _lib.check(L.snapgpu_spectral_apply(
                 X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), self.n,

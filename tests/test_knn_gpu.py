@@ -93,3 +93,23 @@ def test_second_tier_on_offset_clusters(torch_gpu):
     g, st = nearest_neighbour_graph(X, 50, precision="bf16", return_stats=True)
     _check(g, X, 50)
     assert st["fallback_rows"] <= 30
+
+
+@pytest.mark.parametrize("n_clusters,spread", [(10, 3.0), (40, 8.0), (1, 0.0)])
+def test_chunk_skipping_is_exact(torch_gpu, n_clusters, spread):
+    """The bf16 path skips database chunks whose bounding sphere lies beyond every
+    threshold of a workgroup: the result equals the FP64 oracle and the run that
+    visits every chunk; chunks of well-separated clusters are skipped."""
+    from snapatac2_amd.pp._knn import nearest_neighbour_graph
+    rng = np.random.default_rng(n_clusters)
+    n, d, k = 20000, 30, 50
+    centres = spread * rng.standard_normal((n_clusters, d))
+    X = centres[rng.integers(0, n_clusters, n)] + rng.standard_normal((n, d))
+    g, st = nearest_neighbour_graph(X, k, return_stats=True)
+    g_all, st_all = nearest_neighbour_graph(X, k, return_stats=True, prune=False)
+    _check(g, X, k)
+    assert np.array_equal(g.indices, g_all.indices) and np.array_equal(g.data, g_all.data)
+    assert st_all["chunk_visits"] == st_all["chunk_pairs"]
+    assert st["chunk_visits"] <= st["chunk_pairs"]
+    if n_clusters == 40:
+        assert st["chunk_visits"] < 0.7 * st["chunk_pairs"], st

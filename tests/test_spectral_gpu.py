@@ -155,3 +155,61 @@ def test_panel_segmented_operator(torch_gpu):
     # bitwise reproducible
     Vt = torch.from_numpy(rng.standard_normal((70000, 16))).cuda()
     assert torch.equal(op_p.matmat(Vt), op_p.matmat(Vt))
+
+
+@pytest.mark.parametrize("kind", ["small", "float", "large_counts", "empty_rows"])
+def test_panel_layout_variants(torch_gpu, kind):
+    """Panel layout on one panel (forced) and on several: packed counts, float
+    values and counts >= 65536 (unpacked), rows and columns without entries."""
+    torch = torch_gpu
+    from snapatac2_amd.tl._operator import SpectralOperator, _to_device_csr
+    n, m = (3000, 5000) if kind == "small" else (40000, 36000)
+    X = small_counts(n, m, 0.004 if kind != "small" else 0.02, seed=21)
+    if kind == "float":
+        X = X.astype(np.float32)
+        X.data *= np.float32(0.37)
+    elif kind == "large_counts":
+        # whole rows scaled to counts >= 65536 (the unpacked encoding); row
+        # normalisation keeps the operator on the scale of the other cases
+        X = X.astype(np.uint32)
+        X.data[X.indptr[0]:X.indptr[50]] *= 70000
+    elif kind == "empty_rows":
+        X = X.tolil()
+        X[100:140, :] = 0
+        X[:, 33000:34000] = 0
+        X = sp.csr_matrix(X, dtype=np.uint32)
+        X.eliminate_zeros()
+    Xd = _to_device_csr(X, torch)
+    op = SpectralOperator(Xd, use_panels=True)
+    ref_op = SpectralOperator(Xd, use_panels=False)
+    assert op.panel_csr.packed == (kind in ("small", "empty_rows"))
+    Xh, dinv, _, _ = osp.operator_parts(X)
+    rng = np.random.default_rng(3)
+    for b in (16, 7, 32):
+        V = rng.standard_normal((n, b))
+        Vt = torch.from_numpy(V).cuda()
+        Y = op.matmat(Vt).cpu().numpy()
+        ref = Xh @ (Xh.T @ V) - dinv[:, None] * V
+        scale = np.abs(ref).max()
+        assert np.abs(Y - ref).max() / scale < 1e-5, (kind, b)
+        assert np.abs(Y - ref_op.matmat(Vt).cpu().numpy()).max() / scale < 1e-5, (kind, b)
+
+
+def test_warm_start_after_weight_change(torch_gpu):
+    """eigsh_lm started from the eigenvectors of nearby weights finds the same
+    eigenpairs (to the solver tolerance) as a random start."""
+    torch = torch_gpu
+    from snapatac2_amd.tl._lanczos import eigsh_lm
+    from snapatac2_amd.tl._operator import SpectralOperator, _to_device_csr
+    X = small_counts(6000, 4000, 0.02, seed=31)
+    op = SpectralOperator(_to_device_csr(X, torch))
+    _, U0, _ = eigsh_lm(op, 12, seed=0)
+    w = op.w.clone() * torch.from_numpy(np.random.default_rng(2).uniform(0.97, 1.03, op.m)).cuda()
+    op.set_weights(w)
+    ev_r, U_r, st_r = eigsh_lm(op, 12, seed=0)
+    ev_w, U_w, st_w = eigsh_lm(op, 12, seed=0, v0=U0)
+    np.testing.assert_allclose(ev_w, ev_r, rtol=1e-7, atol=1e-9)
+    oe, ov = np.linalg.eigh(osp.dense_operator(X, feature_weights=w.cpu().numpy()))
+    idx = np.lexsort((-oe, -np.abs(oe)))[:12]
+    np.testing.assert_allclose(ev_w, oe[idx], rtol=1e-5, atol=1e-7)
+    assert osp.subspace_angle(U_w.cpu().numpy(), ov[:, idx]) < 1e-4
