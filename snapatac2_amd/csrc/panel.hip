@@ -30,8 +30,15 @@ namespace panel {
 using spectral::kB;
 using spectral::scale_in_kernel;
 
-constexpr int64_t kPanel = 32768;  // dense rows per panel
-constexpr int64_t kChunk = 1024;   // entries per group-chunk (aligned down to pair starts)
+#ifndef SNAPGPU_PANEL_ROWS
+#define SNAPGPU_PANEL_ROWS 32768
+#endif
+#ifndef SNAPGPU_PANEL_CHUNK
+#define SNAPGPU_PANEL_CHUNK 1024
+#endif
+constexpr int64_t kPanel = SNAPGPU_PANEL_ROWS;  // dense rows per panel (<= 32768: 15-bit column)
+constexpr int64_t kChunk = SNAPGPU_PANEL_CHUNK;  // entries per group-chunk (aligned to pair starts)
+static_assert(kPanel <= 32768, "the packed entry holds a 15-bit column");
 
 // seg[row][p] = offset within the row of its first entry with idx >= p * kPanel
 __global__ void __launch_bounds__(256) segment_kernel(const int64_t* __restrict__ ptr,
