@@ -250,7 +250,7 @@ int snapgpu_spectral_vjp_w(const int64_t* d_indptr, const int32_t* d_idx, const 
  * The matrix (CSR of the cells or CSC of the features) is re-laid out
  * panel-major: panel p holds the entries whose column lies in
  * [p * 32768, (p + 1) * 32768), so each product gathers from an L2-sized slice
- * of the dense operand; work is split into chunks of ~1024 entries aligned to
+ * of the dense operand; work is split into chunks of ~4096 entries aligned to
  * (row, panel) segments ("pairs"), whose FP32 sums are then added per row in
  * panel order. Counts below 65536 are packed with the column into 4 bytes.
  * Build: snapgpu_panel_sizes, then allocate, then snapgpu_panel_build with the

@@ -34,7 +34,7 @@ using spectral::scale_in_kernel;
 #define SNAPGPU_PANEL_ROWS 32768
 #endif
 #ifndef SNAPGPU_PANEL_CHUNK
-#define SNAPGPU_PANEL_CHUNK 1024
+#define SNAPGPU_PANEL_CHUNK 4096
 #endif
 constexpr int64_t kPanel = SNAPGPU_PANEL_ROWS;  // dense rows per panel (<= 32768: 15-bit column)
 constexpr int64_t kChunk = SNAPGPU_PANEL_CHUNK;  // entries per group-chunk (aligned to pair starts)
