@@ -1,0 +1,126 @@
+"""Parity at the benchmark sizes, through checks that do not need the oracle on
+the whole input:
+
+- tile binning at 100,000 generator cells: every sampled row bit-exact against
+  the C++ oracle run on those cells' fragments;
+- spectral at 100,000 cells (top 500,000 features): residuals of the 30
+  eigenpairs against an independent FP64 operator built with torch's own sparse
+  products, and orthonormality;
+- kNN at 1,000,000 points (10 Gaussian clusters, row-normalised, 30 dims):
+  sampled rows identical to FP64 brute force computed with torch, and the BF16
+  and FP32 candidate paths giving the same graph.
+"""
+import numpy as np
+import pytest
+
+from oracle import adjoint as oadj
+from oracle import tile as otile
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows(torch, X, rows):
+    """Rows of a DeviceCSR as (indices, values) host arrays."""
+    ip = X.indptr
+    out = []
+    for r in rows:
+        a, b = int(ip[r].item()), int(ip[r + 1].item())
+        idx = X.indices[a:b].cpu().numpy().astype(np.int64)
+        val = X.data[a:b].cpu().numpy().view(np.uint32)
+        out.append((idx, val))
+    return out
+
+
+def test_tile_100k_sampled_rows(torch_gpu):
+    torch = torch_gpu
+    from snapatac2_amd import synth
+    from snapatac2_amd.genome import synthetic_genome
+    from snapatac2_amd.pp import make_tile_matrix_chunked
+    n = 100_000
+    cs = synthetic_genome()
+    chunks = list(synth.iter_chunks(n, seed=7))
+    X, names = make_tile_matrix_chunked(chunks, cs, 500, ["chrM", "chrY", "M", "Y"])
+    assert X.shape == (n, len(names))
+    rng = np.random.default_rng(7)
+    sample = np.sort(rng.choice(n, 2000, replace=False))
+    # fragments of the sampled cells, on the host
+    cell_ptr = [0]
+    pos, ln = [], []
+    for c0, fr in chunks:
+        cp = fr.cell_ptr.cpu().numpy()
+        sel = sample[(sample >= c0) & (sample < c0 + fr.shape[0])] - c0
+        P = fr.pos.cpu().numpy().view(np.uint32)
+        Lv = fr.values.cpu().numpy().view(np.uint32)
+        for s in sel:
+            a, b = int(cp[s]), int(cp[s + 1])
+            pos.append(P[a:b])
+            ln.append(Lv[a:b])
+            cell_ptr.append(cell_ptr[-1] + (b - a))
+    O = otile.tile_matrix(np.asarray(cell_ptr, dtype=np.uint64), np.concatenate(pos),
+                          np.concatenate(ln), cs, 500, ["chrY"])
+    got = _rows(torch, X, sample)
+    for k, (idx, val) in enumerate(got):
+        a, b = O.indptr[k], O.indptr[k + 1]
+        assert np.array_equal(idx, O.indices[a:b]), sample[k]
+        assert np.array_equal(val, O.data[a:b]), sample[k]
+
+
+def test_spectral_100k_residuals(torch_gpu):
+    torch = torch_gpu
+    from snapatac2_amd import synth
+    from snapatac2_amd.genome import synthetic_genome
+    from snapatac2_amd.pp import make_tile_matrix_chunked
+    from snapatac2_amd.pp._basic import _column_sums, _find_most_accessible_features
+    from snapatac2_amd.tl._lanczos import eigsh_lm
+    from snapatac2_amd.tl._operator import SpectralOperator, select_columns
+    n = 100_000
+    chunks = list(synth.iter_chunks(n, seed=0))
+    X, _ = make_tile_matrix_chunked(chunks, synthetic_genome(), 500, ["chrM", "chrY"])
+    del chunks
+    sel = np.sort(_find_most_accessible_features(_column_sums(X), 0.005, 0.005, 500_000))
+    Xs = select_columns(X, sel)
+    del X
+    op = SpectralOperator(Xs)
+    assert op.use_panels  # 500,000 features: more than one panel
+    evals, U, st = eigsh_lm(op, 30, seed=0)
+    # independent FP64 operator from the same counts (torch's sparse products)
+    rows = torch.repeat_interleave(torch.arange(n, device="cuda"), torch.diff(Xs.indptr))
+    vals = Xs.data.to(torch.int64).to(torch.float64)
+    T = oadj.TorchCounts(rows, Xs.indices.long(), vals, (n, op.m))
+    del rows, vals
+    AU = oadj.apply_tensors(T, op.w, U)
+    lam = torch.from_numpy(np.asarray(evals)).to(U.device)
+    res = torch.linalg.norm(AU - U * lam, dim=0)
+    scale = float(np.abs(evals).max())
+    # FP32 sparse products inside the solver: the FP64 operator differs by ~1e-7
+    assert float(res.max()) < 1e-5 * scale, (res.max().item(), scale)
+    G = U.T @ U
+    assert float((G - torch.eye(30, dtype=G.dtype, device=G.device)).abs().max()) < 1e-10
+    # Rayleigh quotients of the FP64 operator match the eigenvalues
+    rq = (U * AU).sum(0)
+    assert float(((rq - lam).abs() / lam.abs()).max()) < 1e-5
+
+
+def test_knn_1m_clusters_sampled_rows(torch_gpu):
+    torch = torch_gpu
+    from snapatac2_amd.pp._knn import nearest_neighbour_graph
+    n, d, k = 1_000_000, 30, 50
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    centres = 3.0 * torch.randn(10, d, dtype=torch.float64, device="cuda", generator=gen)
+    lab = torch.randint(0, 10, (n,), device="cuda", generator=gen)
+    X = centres[lab] + torch.randn(n, d, dtype=torch.float64, device="cuda", generator=gen)
+    X = (X / torch.linalg.norm(X, dim=1, keepdim=True)).to(torch.float32).to(torch.float64)
+    g16, st16 = nearest_neighbour_graph(X, k, precision="bf16", return_stats=True)
+    g32, st32 = nearest_neighbour_graph(X, k, precision="fp32", return_stats=True)
+    assert np.array_equal(g16.indices, g32.indices) and np.array_equal(g16.data, g32.data)
+    rng = np.random.default_rng(1)
+    qi = rng.choice(n, 200, replace=False)
+    for q in qi:
+        d2 = ((X - X[q]) ** 2).sum(1)
+        d2[q] = float("inf")
+        order = np.lexsort((np.arange(n), d2.cpu().numpy()))[:k]
+        exact = np.sort(order)
+        got = g16.indices[g16.indptr[q]:g16.indptr[q + 1]]
+        assert np.array_equal(got, exact), q
+        np.testing.assert_allclose(g16.data[g16.indptr[q]:g16.indptr[q + 1]],
+                                   np.sqrt(d2.cpu().numpy()[exact]), rtol=1e-12)
