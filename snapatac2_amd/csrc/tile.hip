@@ -18,9 +18,10 @@
 // lower bound T for all future events is known, every merged (bin, count) below
 // T is final and is streamed out; bins >= T wait in a small LDS "pending" list
 // (at most max-span-in-bins + 1 entries). Within a tile the <= 128 new events
-// are put in order by a register bitonic network over the wave, run-length
-// encoded with ballots, and merged with the pending list by rank (binary
-// search in LDS). The same code runs twice: pass 1 only counts, pass 2 writes.
+// are put in order (directly by rank when the fragments' bins do not
+// interleave, the usual case, else by a register bitonic network over the
+// wave), run-length encoded with ballots, and merged with the pending list by
+// rank (binary search in LDS). The same code runs twice: pass 1 only counts, pass 2 writes.
 #include "common.h"
 
 namespace snapgpu {
@@ -172,16 +173,46 @@ __device__ __forceinline__ int lds_rank(const uint32_t* arr, int n, uint32_t key
   return lo;
 }
 
+// Fast path for the per-fragment event pairs (k[0] = first event of lane i's
+// fragment, k[1] = its second event or kInf; kInf in k[0] = no event). Start
+// bins never decrease along a row and the second event is >= the first, so the
+// pair order (lane 0 first, lane 0 second, lane 1 first, ...) is sorted unless
+// some fragment ends in a later bin than where the next fragment starts. When
+// it is sorted, the events are placed at their ranks (elements 0..63 in k[0],
+// 64..127 in k[1], kInf last) and true is returned; otherwise k is unchanged.
+template <int CAP>
+__device__ __forceinline__ bool presort_pairs(uint32_t (&k)[2], int lane, WaveLds<CAP>* L) {
+  const bool v0 = k[0] != kInf, v1 = k[1] != kInf;
+  const uint64_t m0 = ballot64(v0), m1 = ballot64(v1);
+  const uint64_t later = m0 & lanemask_gt(lane);
+  const int nl = later ? lowbit64(later) : lane;
+  const uint32_t nxt = __shfl(k[0], nl, 64);
+  const uint32_t last = v1 ? k[1] : k[0];
+  if (ballot64(v0 && later != 0 && last > nxt) != 0) return false;
+  const uint64_t lt = lanemask_lt(lane);
+  const int r0 = popc64(m0 & lt) + popc64(m1 & lt);
+  L->nbin[lane] = kInf;
+  L->nbin[64 + lane] = kInf;
+  wave_sync();
+  if (v0) L->nbin[r0] = k[0];
+  if (v1) L->nbin[r0 + 1] = k[1];
+  wave_sync();
+  k[0] = L->nbin[lane];
+  k[1] = L->nbin[64 + lane];
+  wave_sync();
+  return true;
+}
+
 // Merges one round of up to 128 new unit events with the pending list, streams out
 // every merged bin below T and keeps the rest pending.
 template <int CAP, bool FILL, typename IdxT>
 __device__ void merge_round(CellState<CAP, FILL, IdxT>& s, const Params& p, uint32_t (&k)[2],
-                            uint32_t T, int lane) {
+                            uint32_t T, int lane, bool sorted = false) {
   WaveLds<CAP>* L = s.L;
   const uint64_t any = ballot64(k[0] != kInf) | ballot64(k[1] != kInf);
   if (any == 0 && s.np == 0) return;
 
-  bitonic128(k, lane);
+  if (!sorted) bitonic128(k, lane);
 
   // Run-length encode the sorted events (element order: k[0] lanes 0..63, then k[1]).
   const uint32_t prev0 = __shfl_up(k[0], 1, 64);
@@ -338,7 +369,8 @@ __device__ void process_cell(const Params& p, const ChromRec* tab, WaveLds<CAP>*
       if (p.strategy == SNAPGPU_INSERTION) k[1] = kept ? b2 : kInf;
       else if (p.strategy == SNAPGPU_PIC) k[1] = (kept && b2 != b1) ? b2 : kInf;
       else k[1] = kInf;
-      merge_round(s, p, k, T, lane);
+      const bool sorted = presort_pairs(k, lane, L);
+      merge_round(s, p, k, T, lane, sorted);
       if (s.overflow) break;
     } else {
       // Every bin b1..=b2 gets +1. The events of the tile are laid out lane by

@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import logging
+import threading
 from typing import Literal
 
 import numpy as np
@@ -108,6 +109,9 @@ def make_tile_matrix(
     return DeviceCSR(indptr, out_idx, out_val, (n, n_bins)), BinIndex(kept, bin_size)
 
 
+_TILE_STREAMS = 4
+
+
 def make_tile_matrix_chunked(chunks, chrom_sizes: dict[str, int], bin_size: int = 500,
                              exclude_chroms: list[str] | None = None,
                              min_frag_size: int | None = None, max_frag_size: int | None = None,
@@ -134,17 +138,51 @@ def make_tile_matrix_chunked(chunks, chrom_sizes: dict[str, int], bin_size: int 
     min_len = 0 if min_frag_size is None else int(min_frag_size)
     max_len = 0xFFFFFFFF if max_frag_size is None else int(max_frag_size)
     indptr = torch.zeros(n + 1, dtype=torch.int64, device=dev)
-    status = torch.zeros(8, dtype=torch.int32, device=dev)
-    stream = _lib.stream_ptr()
-    levels = []
-    for c0, fr in chunks:
+    main = torch.cuda.current_stream()
+    # Chunks are counted and filled by a few host threads, each on its own
+    # stream: every library call waits for its own launch (to read the kernel's
+    # status), so one thread would leave the GPU idle behind the longest cells
+    # of each chunk; several streams overlap those tails.
+    n_streams = min(_TILE_STREAMS, len(chunks))
+    streams = [torch.cuda.Stream(device=dev) for _ in range(n_streams)]
+    statuses = [torch.zeros(8, dtype=torch.int32, device=dev) for _ in range(n_streams)]
+    for st_ in streams:
+        st_.wait_stream(main)
+    levels = [0] * len(chunks)
+
+    def run(fn):
+        errors = []
+
+        def worker(k):
+            try:
+                with torch.cuda.stream(streams[k]):
+                    for ci in range(k, len(chunks), n_streams):
+                        fn(ci, statuses[k], _lib.stream_ptr())
+            except BaseException as exc:  # re-raised in the calling thread
+                errors.append(exc)
+
+        threads = [threading.Thread(target=worker, args=(k,)) for k in range(n_streams)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+        for st_ in streams:
+            main.wait_stream(st_)
+
+    def count(ci, status, stream):
+        c0, fr = chunks[ci]
         level = C.c_int32(0)
         _lib.check(L.snapgpu_tile_count(fr.cell_ptr.data_ptr(), fr.pos.data_ptr(),
                                         _lib.ptr(fr.values) if fr.paired else 0, fr.shape[0],
                                         table.data_ptr(), n_chrom, int(bin_size), strategy,
                                         min_len, max_len, indptr.data_ptr() + 8 * (c0 + 1),
                                         status.data_ptr(), C.byref(level), stream))
-        levels.append(level.value)
+        levels[ci] = level.value
+
+    run(count)
+    stream = _lib.stream_ptr()
     ws_b = int(L.snapgpu_scan_workspace_bytes(n))
     ws = torch.empty(max(ws_b, 8), dtype=torch.uint8, device=dev)
     _lib.check(L.snapgpu_scan_inclusive_i64(indptr.data_ptr() + 8, n, ws.data_ptr(), ws_b, stream))
@@ -152,13 +190,19 @@ def make_tile_matrix_chunked(chunks, chrom_sizes: dict[str, int], bin_size: int 
     idx_bits = 32 if n_bins < 2**31 else 64
     out_idx = torch.empty(nnz, dtype=torch.int32 if idx_bits == 32 else torch.int64, device=dev)
     out_val = torch.empty(nnz, dtype=torch.int32, device=dev)
-    for (c0, fr), lv in zip(chunks, levels):
+    for st_ in streams:
+        st_.wait_stream(main)
+
+    def fill(ci, status, stream):
+        c0, fr = chunks[ci]
         _lib.check(L.snapgpu_tile_fill(fr.cell_ptr.data_ptr(), fr.pos.data_ptr(),
                                        _lib.ptr(fr.values) if fr.paired else 0, fr.shape[0],
                                        table.data_ptr(), n_chrom, int(bin_size), strategy,
-                                       min_len, max_len, lv, indptr.data_ptr() + 8 * c0,
+                                       min_len, max_len, levels[ci], indptr.data_ptr() + 8 * c0,
                                        out_idx.data_ptr(), idx_bits, out_val.data_ptr(),
                                        status.data_ptr(), stream))
+
+    run(fill)
     kept = {c: chrom_sizes[c] for c in names if c not in excl}
     return DeviceCSR(indptr, out_idx, out_val, (n, n_bins)), BinIndex(kept, bin_size)
 
