@@ -318,11 +318,15 @@ static void launch_apply(const int64_t* indptr, const int32_t* idx, const uint32
 // ---------------------------------------------------------------------------
 // FP64 tall-skinny products for the block Lanczos basis
 // ---------------------------------------------------------------------------
-constexpr int kTnRows = 32;
-// partial[chunk][a][c] = sum_{i in chunk} A[i][a0 + a] * B[i][c]  (a < 32, c < kb <= 32)
-__global__ void __launch_bounds__(256) gemm_tn_kernel(const double* A, int64_t lda, const double* B,
-                                                      int64_t ldb, int64_t n, int ka, int kb,
-                                                      int64_t rows_per_chunk, double* partial) {
+constexpr int kTnRows = 64;
+// partial[chunk][a][c] = sum_{i in chunk} A[i][a0 + a] * B[i][c]  (a < 32, c < kb <= 32).
+// Tiles of 64 rows: the next tile's 16 values per thread are loaded into
+// registers while the current tile is multiplied out of LDS, so the global
+// loads of a block overlap its arithmetic. Rows are added in ascending order.
+__global__ void __launch_bounds__(256) gemm_tn_kernel(const double* __restrict__ A, int64_t lda,
+                                                      const double* __restrict__ B, int64_t ldb, int64_t n,
+                                                      int ka, int kb, int64_t rows_per_chunk,
+                                                      double* __restrict__ partial) {
   __shared__ double As[kTnRows][33];
   __shared__ double Bs[kTnRows][33];
   const int64_t chunk = blockIdx.x;
@@ -332,21 +336,35 @@ __global__ void __launch_bounds__(256) gemm_tn_kernel(const double* A, int64_t l
   const int oc = (t & 7) * 4;   // 4 output columns
   double acc[4] = {0, 0, 0, 0};
   const int64_t r0 = chunk * rows_per_chunk, r1 = min(n, r0 + rows_per_chunk);
+  // element e of a tile (e = k * 256 + t, k < 8): row e >> 5, column e & 31
+  double va[8], vb[8];
+  auto load = [&](int64_t rb) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int e = k * 256 + t;
+      const int64_t i = rb + (e >> 5);
+      const int cc = e & 31;
+      va[k] = (i < r1 && a0 + cc < ka) ? A[i * lda + a0 + cc] : 0.0;
+      vb[k] = (i < r1 && cc < kb) ? B[i * ldb + cc] : 0.0;
+    }
+  };
+  if (r0 < r1) load(r0);
   for (int64_t rb = r0; rb < r1; rb += kTnRows) {
-    for (int q = t; q < kTnRows * 32; q += 256) {
-      const int rr = q >> 5, cc = q & 31;
-      const int64_t i = rb + rr;
-      As[rr][cc] = (i < r1 && a0 + cc < ka) ? A[i * lda + a0 + cc] : 0.0;
-      Bs[rr][cc] = (i < r1 && cc < kb) ? B[i * ldb + cc] : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int e = k * 256 + t;
+      As[e >> 5][e & 31] = va[k];
+      Bs[e >> 5][e & 31] = vb[k];
     }
     __syncthreads();
+    if (rb + kTnRows < r1) load(rb + kTnRows);
 #pragma unroll 8
     for (int rr = 0; rr < kTnRows; ++rr) {
       const double av = As[rr][oa];
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[q] = fma(av, Bs[rr][oc + q], acc[q]);
     }
-    __syncthreads();
   }
   double* P = partial + chunk * (int64_t)ka * kb;
 #pragma unroll
@@ -377,13 +395,24 @@ __global__ void __launch_bounds__(256) gemm_nn_kernel(const double* __restrict__
   const int r = t >> 2, cg = t & 3;
   const int64_t row0 = (int64_t)blockIdx.x * kNnRows;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int a0 = 0; a0 < ka; a0 += 32) {
+  // element e of an A slice (e = k * 256 + t): row e >> 5, column a0 + (e & 31);
+  // the next slice is loaded into registers while the current one is used
+  double va[8];
+  auto load = [&](int a0) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int e = k * 256 + t;
-      const int rr = e >> 5, cc = e & 31;
-      const int64_t i = row0 + rr;
-      As[rr][cc] = (i < n && a0 + cc < ka) ? A[i * lda + a0 + cc] : 0.0;
+      const int64_t i = row0 + (e >> 5);
+      va[k] = (i < n && a0 + (e & 31) < ka) ? A[i * lda + a0 + (e & 31)] : 0.0;
+    }
+  };
+  load(0);
+  for (int a0 = 0; a0 < ka; a0 += 32) {
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int e = k * 256 + t;
+      As[e >> 5][e & 31] = va[k];
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -392,13 +421,13 @@ __global__ void __launch_bounds__(256) gemm_nn_kernel(const double* __restrict__
       Bs[ra][cc] = (a0 + ra < ka && cc < kb) ? B[(int64_t)(a0 + ra) * kb + cc] : 0.0;
     }
     __syncthreads();
+    if (a0 + 32 < ka) load(a0 + 32);
     const int lim = min(32, ka - a0);
     for (int ra = 0; ra < lim; ++ra) {
       const double av = As[r][ra];
 #pragma unroll
       for (int j = 0; j < 8; ++j) acc[j] = fma(av, Bs[ra][cg + 4 * j], acc[j]);
     }
-    __syncthreads();
   }
   const int64_t i = row0 + r;
   if (i < n) {
