@@ -111,10 +111,20 @@ __global__ void scatter_chunk_kernel(const uint32_t* skeys, const uint32_t* spos
   }
 }
 
+// cursor[col] += entries of col in this chunk: the last element of each run of
+// equal keys adds the run's length (one atomic per column present).
 __global__ void cursor_advance_kernel(const uint32_t* keys, int64_t cn, int64_t* cursor) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cn; t += stride)
-    atomicAdd(reinterpret_cast<unsigned long long*>(&cursor[keys[t]]), 1ull);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cn; t += stride) {
+    const uint32_t col = keys[t];
+    if (t + 1 < cn && keys[t + 1] == col) continue;
+    int64_t lo = 0, hi = t;  // first position with key == col
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (keys[mid] < col) lo = mid + 1; else hi = mid;
+    }
+    atomicAdd(reinterpret_cast<unsigned long long*>(&cursor[col]), (unsigned long long)(t - lo + 1));
+  }
 }
 
 // ---------------------------------------------------------------------------
