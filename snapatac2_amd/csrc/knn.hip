@@ -6,16 +6,20 @@
 // snapatac2-python/src/knn.rs:7-16.
 //
 // Pipeline (all on the device):
-//  1. prepare: FP64 rows -> augmented 32-wide rows x~ = (x, |x|^2, 0) for the
-//     database and q~ = (x, -1/2, 0) for the queries, so s = x~.q~ = x.q - |x|^2/2
-//     and d^2 = |q|^2 - 2 s. Stored as three bf16 terms (x = hi + mid + lo +
-//     O(2^-24 x)) and as fp32.
-//  2. candidates: a wave owns 16 queries and streams every database point in
-//     tiles of 16. BF16 mode: six v_mfma_f32_16x16x32_bf16 per tile (every
-//     product of the split terms down to order 2^-24, fp32 accumulation), so the
-//     approximation error is at fp32 level even where d^2 is much smaller than
-//     |q|^2 (spectral embeddings); FP32 mode: SIMT FMAs. Output
-//     lane l holds query l%16 and 4 points of the tile, so the 4 lanes of a query
+//  1. prepare: FP64 rows -> augmented 32-wide fp32 rows x~ = (x, |x|^2, 0) for
+//     the database and q~ = (x, -1/2, 0) for the queries, so s = x~.q~ =
+//     x.q - |x|^2/2 and d^2 = |q|^2 - 2 s; |x|^2 and the largest norm.
+//  2. candidates. BF16 mode: the points are put in kd-tree leaf order (leaves of
+//     128), each leaf is shifted by its own centre, and a workgroup of 8 waves
+//     (128 queries = one leaf) streams database leaves through LDS, each as
+//     three bf16 terms (x = hi + mid + lo + O(2^-24 x)) multiplied with six
+//     v_mfma_f32_16x16x32_bf16 per 16x16 tile (every product of the split terms
+//     down to order 2^-24, fp32 accumulation), so the approximation error is at
+//     fp32 level even where d^2 is much smaller than |q|^2. A leaf whose
+//     bounding sphere lies beyond every query's current threshold is skipped
+//     (the threshold is kept in the certificate, so skipping stays exact).
+//     FP32 mode: SIMT FMAs over every point in the original order. In both,
+//     lane l holds query l%16 and 4 points of a tile, so the 4 lanes of a query
 //     cover 4 disjoint stripes of the database; each lane keeps its best 32
 //     (sorted in registers) -> 128 candidates per query (>= 2k for k <= 64).
 //  3. finalize: one wave per query rescoring its candidates in exact FP64
@@ -47,7 +51,7 @@ constexpr int kList = 32;     // candidates kept per lane (stripe)
 constexpr int kCand = 4 * kList;  // candidates per query
 constexpr int kMaxK = 100;
 
-enum Mode : int32_t { kModeFP32 = 0, kModeBF16 = 1, kModeExact = 2 };
+enum Mode : int32_t { kModeFP32 = 0, kModeBF16 = 1 };
 
 // Error constants c of the approximate d^2 (see the certificate above):
 // |approx - exact| <= c * (|x||q| + |x|^2 + |q|^2).
@@ -65,10 +69,9 @@ __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
 // 1. prepare
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) prepare_kernel(const double* __restrict__ X, int64_t n,
-                                                      int d, int64_t ld, int64_t n_pad,
-                                                      __bf16* xh, __bf16* xm, __bf16* xl, __bf16* qh,
-                                                      __bf16* qm, __bf16* ql, float* xf, float* qf,
-                                                      float* qnorm, unsigned int* max_norm_bits) {
+                                                      int d, int64_t ld, int64_t n_pad, float* xf,
+                                                      float* qf, float* qnorm,
+                                                      unsigned int* max_norm_bits) {
   const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
   const int c = threadIdx.x & 31;
   if (row >= n_pad) return;
@@ -88,19 +91,8 @@ __global__ void __launch_bounds__(256) prepare_kernel(const double* __restrict__
   }
   if (row >= n) qv = 0.0;
   const int64_t o = row * kDim + c;
-  const float xv32 = (float)xv, qv32 = (float)qv;
-  // three-term split: each step keeps the next 8 significant bits
-  const __bf16 xhv = to_bf16(xv32), qhv = to_bf16(qv32);
-  const float xr1 = xv32 - (float)xhv, qr1 = qv32 - (float)qhv;
-  const __bf16 xmv = to_bf16(xr1), qmv = to_bf16(qr1);
-  xh[o] = xhv;
-  xm[o] = xmv;
-  xl[o] = to_bf16(xr1 - (float)xmv);
-  qh[o] = qhv;
-  qm[o] = qmv;
-  ql[o] = to_bf16(qr1 - (float)qmv);
-  xf[o] = xv32;
-  qf[o] = qv32;
+  xf[o] = (float)xv;
+  qf[o] = (float)qv;
   if (c == 0) {
     qnorm[row] = (float)sq;
     if (row < n) atomicMax(max_norm_bits, __float_as_uint((float)sqrt(sq) * 1.0000001f));
@@ -111,11 +103,7 @@ __global__ void __launch_bounds__(256) prepare_kernel(const double* __restrict__
 // 2. candidates
 // ---------------------------------------------------------------------------
 struct CandParams {
-  const __bf16 *xh, *xm, *xl, *qh, *qm, *ql;
   const float *xf, *qf, *qnorm;
-  const double* X;  // FP64 rows (exact mode)
-  int64_t ld;
-  int d;
   int64_t n, n_pad;
   const int32_t* rows;  // optional list of query rows (null = all)
   int64_t n_rows;
@@ -482,7 +470,6 @@ __global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredPara
   if (g == 0) P.qnorm_c[qi] = nqc;
 }
 
-template <int MODE>
 __global__ void __launch_bounds__(256) cand_kernel(CandParams P) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -502,70 +489,7 @@ __global__ void __launch_bounds__(256) cand_kernel(CandParams P) {
   }
   const int64_t ntiles = P.n_pad / 16;
 
-  if (MODE == kModeBF16) {
-    const bf16x8 bq_h = *reinterpret_cast<const bf16x8*>(P.qh + q * kDim + g * 8);
-    const bf16x8 bq_m = *reinterpret_cast<const bf16x8*>(P.qm + q * kDim + g * 8);
-    const bf16x8 bq_l = *reinterpret_cast<const bf16x8*>(P.ql + q * kDim + g * 8);
-    const bf16x8* xh8 = reinterpret_cast<const bf16x8*>(P.xh);
-    const bf16x8* xm8 = reinterpret_cast<const bf16x8*>(P.xm);
-    const bf16x8* xl8 = reinterpret_cast<const bf16x8*>(P.xl);
-    bf16x8 ah = xh8[(int64_t)j * 4 + g];
-    bf16x8 am = xm8[(int64_t)j * 4 + g];
-    bf16x8 al = xl8[(int64_t)j * 4 + g];
-    for (int64_t t = 0; t < ntiles; ++t) {
-      // smallest terms first
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bq_h, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bq_m, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_l, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bq_h, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_m, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_h, acc, 0, 0, 0);
-      if (t + 1 < ntiles) {  // prefetch the next tile's A operands
-        const int64_t row = (t + 1) * 16 + j;
-        ah = xh8[row * 4 + g];
-        am = xm8[row * 4 + g];
-        al = xl8[row * 4 + g];
-      }
-      const float th = L[kList - 1];
-      const int64_t pbase = t * 16 + g * 4;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float s = acc[v];
-        const int64_t p = pbase + v;
-        const bool ins = s > th && p < P.n;
-        if (__ballot(ins)) {
-          if (ins) list_insert(L, I, s, (int32_t)p);
-        }
-      }
-    }
-  } else if (MODE == kModeExact) {
-    // second tier: exact FP64 differences; the key is -fl32(d^2)
-    double qv[kMaxData];
-#pragma unroll
-    for (int c = 0; c < kMaxData; ++c) qv[c] = c < P.d ? P.X[q * P.ld + c] : 0.0;
-    for (int64_t t = 0; t < ntiles; ++t) {
-      const float th = L[kList - 1];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int64_t p = t * 16 + g * 4 + v;
-        float s = -INFINITY;
-        if (p < P.n) {
-          const double* xr = P.X + p * P.ld;
-          double acc = 0.0;
-          for (int c = 0; c < P.d; ++c) {
-            const double df = xr[c] - qv[c];
-            acc = fma(df, df, acc);
-          }
-          s = -(float)acc;
-        }
-        const bool ins = s > th;
-        if (__ballot(ins)) {
-          if (ins) list_insert(L, I, s, (int32_t)p);
-        }
-      }
-    }
-  } else {
+  {
     float qv[kDim];
 #pragma unroll
     for (int c = 0; c < kDim; ++c) qv[c] = P.qf[q * kDim + c];
@@ -596,12 +520,8 @@ __global__ void __launch_bounds__(256) cand_kernel(CandParams P) {
 #pragma unroll
   for (int r = 0; r < kList; ++r) out[r] = I[r];
   // stripe bound on the approximate d^2 of every point this lane did not keep
-  if (MODE == kModeExact) {
-    P.tau[qi * 4 + g] = (L[kList - 1] == -INFINITY) ? INFINITY : -L[kList - 1];
-  } else {
-    const float nq = P.qnorm[q];
-    P.tau[qi * 4 + g] = (L[kList - 1] == -INFINITY) ? INFINITY : nq - 2.0f * L[kList - 1];
-  }
+  const float nq = P.qnorm[q];
+  P.tau[qi * 4 + g] = (L[kList - 1] == -INFINITY) ? INFINITY : nq - 2.0f * L[kList - 1];
 }
 
 // ---------------------------------------------------------------------------
@@ -665,7 +585,6 @@ struct FinParams {
   const float* qnorm_qi;  // if set: |q - centre|^2 by list position (centred BF16 path)
   const unsigned int* max_norm_bits;
   double eps_c;
-  int mode;
   int force_fallback;
   int32_t* nbr;     // [n][k]
   double* dist;     // [n][k]
@@ -753,14 +672,7 @@ __global__ void __launch_bounds__(256) finalize_kernel(FinParams P) {
   for (int g = 1; g < 4; ++g) tmin = fminf(tmin, P.tau[qi * 4 + g]);
   const double M = (double)__uint_as_float(*P.max_norm_bits);
   const double nq = P.qnorm_qi ? (double)P.qnorm_qi[qi] : (double)P.qnorm[q];
-  double lb;
-  if (P.mode == kModeExact) {
-    // keys are fl32 of the exact FP64 d^2 (FMA accumulation; within a few ulp of
-    // the rescored value): non-kept points have exact d^2 >= tau (1 - 2^-20)
-    lb = (double)tmin * (1.0 - 0x1p-20) - 64.0 * DBL_MIN;
-  } else {
-    lb = nonkept_lower_bound(tmin, nq, P.qnorm_qi ? INFINITY : M, P.eps_c);
-  }
+  const double lb = nonkept_lower_bound(tmin, nq, P.qnorm_qi ? INFINITY : M, P.eps_c);
   bool ok = nvalid >= k && (tmin == INFINITY || lb > dk);
   if (P.force_fallback) ok = false;
   if (!ok) {
@@ -994,12 +906,7 @@ __global__ void __launch_bounds__(256) finalize_multi_kernel(FinParams P, SegPar
   }
 }
 
-static inline unsigned grid_knn(int64_t work) {
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, 256), 16384));
-}
-
 struct Workspace {
-  __bf16 *xh, *xm, *xl, *qh, *qm, *ql;
   float *xf, *qf, *qnorm, *tau;
   int32_t *cand, *fail_rows, *fail_rows2;
   unsigned int *max_norm_bits, *fail_count, *fail_count2;
@@ -1024,12 +931,6 @@ static int64_t layout(int64_t n, Workspace* w, char* base) {
     return p;
   };
   Workspace t;
-  t.xh = (__bf16*)take(n_pad * kDim * 2);
-  t.xm = (__bf16*)take(n_pad * kDim * 2);
-  t.xl = (__bf16*)take(n_pad * kDim * 2);
-  t.qh = (__bf16*)take(n_pad * kDim * 2);
-  t.qm = (__bf16*)take(n_pad * kDim * 2);
-  t.ql = (__bf16*)take(n_pad * kDim * 2);
   t.xf = (float*)take(n_pad * kDim * 4);
   t.qf = (float*)take(n_pad * kDim * 4);
   t.qnorm = (float*)take(n_pad * 4);
@@ -1101,12 +1002,11 @@ int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, 
   SG_HIP(hipMemsetAsync(w.max_norm_bits, 0, 4, st));
   SG_HIP(hipMemsetAsync(w.fail_count, 0, 4, st));
   prepare_kernel<<<(unsigned)ceil_div(n_pad * 32, 256), 256, 0, st>>>(
-      d_X, n, d, ld, n_pad, w.xh, w.xm, w.xl, w.qh, w.qm, w.ql, w.xf, w.qf, w.qnorm, w.max_norm_bits);
+      d_X, n, d, ld, n_pad, w.xf, w.qf, w.qnorm, w.max_norm_bits);
   SG_LAUNCH_CHECK();
-  CandParams cp{w.xh, w.xm, w.xl, w.qh, w.qm, w.ql, w.xf, w.qf, w.qnorm, d_X, ld, d,
-                n, n_pad, nullptr, n, w.cand, w.tau};
+  CandParams cp{w.xf, w.qf, w.qnorm, n, n_pad, nullptr, n, w.cand, w.tau};
   FinParams fp{d_X, n, ld, d, k, nullptr, n, w.cand, w.tau, w.qnorm, nullptr, w.max_norm_bits,
-               mode == kModeBF16 ? kEpsBF16 : kEpsFP32, mode, (flags & 1) ? 1 : 0,
+               mode == kModeBF16 ? kEpsBF16 : kEpsFP32, (flags & 1) ? 1 : 0,
                d_nbr, d_dist, w.fail_rows, w.fail_count};
   if (mode == kModeBF16) {
     // spatial order of the queries, per-workgroup centres, centred LDS-staged MFMA tiles
@@ -1160,7 +1060,7 @@ int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, 
     fp.qnorm_qi = w.qnorm_c;
   } else {
     const unsigned cblocks = (unsigned)ceil_div(ceil_div(n, 16), 4);
-    cand_kernel<kModeFP32><<<cblocks, 256, 0, st>>>(cp);
+    cand_kernel<<<cblocks, 256, 0, st>>>(cp);
     SG_LAUNCH_CHECK();
   }
   finalize_kernel<<<(unsigned)ceil_div(n, 4), 256, 0, st>>>(fp);
