@@ -328,126 +328,297 @@ static void launch_apply(const int64_t* indptr, const int32_t* idx, const uint32
 // ---------------------------------------------------------------------------
 // FP64 tall-skinny products for the block Lanczos basis
 // ---------------------------------------------------------------------------
-constexpr int kTnRows = 64;
-// partial[chunk][a][c] = sum_{i in chunk} A[i][a0 + a] * B[i][c]  (a < 32, c < kb <= 32).
-// Tiles of 64 rows: the next tile's 16 values per thread are loaded into
-// registers while the current tile is multiplied out of LDS, so the global
-// loads of a block overlap its arithmetic. Rows are added in ascending order.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// The products below run on v_mfma_f64_16x16x4_f64 (D[16x16] += A[16x4] B[4x16],
+// FP64 in and out). Lane l supplies A[l%16][l/16] and B[l/16][l%16] and holds
+// D[l/16 + 4r][l%16], r < 4. Which matrix index a lane's operand stands for is
+// free as long as both operands agree on the K index, so every lane can load a
+// contiguous run of one row of the tall matrix.
+
+// --- C = A^T B --------------------------------------------------------------
+// MFMA M = columns of A, N = columns of B, K = rows. A block (4 waves) owns a
+// chunk of rows and 16*TA columns of A; lane l loads TA consecutive columns
+// a0 + TA*(l%16) .. +TA-1 of row (l/16) of each 4-row step (MFMA t then stands
+// for column a0 + TA*i + t of M index i) and one value of B (column l%16 of
+// each 16-wide tile). Rows are clamped to the chunk and their B values zeroed,
+// so every load is unconditional and in bounds.
+constexpr int kTnWaves = 4;
+
+template <int TA, int TC, int U, bool VEC>
+__device__ __forceinline__ void tn_load(const double* __restrict__ A, int64_t lda,
+                                        const double* __restrict__ B, int64_t ldb, int ka, int kb,
+                                        int a_lane, int c_lane, int64_t row0, int64_t r1,
+                                        double (&va)[U][TA], double (&vb)[U][TC]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t row = row0 + 4 * u;
+    const bool rv = row < r1;
+    const int64_t rc = rv ? row : r1 - 1;
+    const double* ap = A + rc * lda + a_lane;
+    if constexpr (VEC) {
+      static_assert(TA % 2 == 0, "vector loads take column pairs");
+#pragma unroll
+      for (int t = 0; t < TA; t += 2) {
+        const double2 v = *reinterpret_cast<const double2*>(ap + t);
+        va[u][t] = v.x;
+        va[u][t + 1] = v.y;
+      }
+    } else {
+      // columns past ka read column ka - 1; they only reach outputs never stored
+#pragma unroll
+      for (int t = 0; t < TA; ++t) va[u][t] = A[rc * lda + min(a_lane + t, ka - 1)];
+    }
+    // rows past the chunk contribute zero; a multiply rather than a select keeps
+    // the load unconditional (a select lets the compiler branch around the load
+    // and wait for it in the branch)
+    const double keep = rv ? 1.0 : 0.0;
+#pragma unroll
+    for (int tc = 0; tc < TC; ++tc) vb[u][tc] = B[rc * ldb + min(16 * tc + c_lane, kb - 1)] * keep;
+  }
+}
+
+template <int TA, int TC, int U>
+__device__ __forceinline__ void tn_mfma(f64x4 (&acc)[TA][TC], const double (&va)[U][TA],
+                                        const double (&vb)[U][TC]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int t = 0; t < TA; ++t)
+#pragma unroll
+      for (int tc = 0; tc < TC; ++tc)
+        acc[t][tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[u][t], vb[u][tc], acc[t][tc], 0, 0, 0);
+}
+
+// Rows of the chunk in steps of 2 * 4 waves * U * 4; two register sets, each
+// loaded one step ahead of its use.
+// lrow = this lane's row offset within a step: wave * U * 4 + l / 16.
+template <int TA, int TC, int U, bool VEC>
+__device__ __forceinline__ void tn_rows(const double* __restrict__ A, int64_t lda,
+                                        const double* __restrict__ B, int64_t ldb, int ka, int kb,
+                                        int a_lane, int c_lane, int64_t r0, int64_t r1, int lrow,
+                                        f64x4 (&acc)[TA][TC]) {
+  constexpr int kStep = kTnWaves * U * 4;
+  double xa[U][TA], xb[U][TC], ya[U][TA], yb[U][TC];
+  tn_load<TA, TC, U, VEC>(A, lda, B, ldb, ka, kb, a_lane, c_lane, r0 + lrow, r1, xa, xb);
+  for (int64_t rb = r0; rb < r1; rb += 2 * kStep) {  // wave-uniform trip count
+    tn_load<TA, TC, U, VEC>(A, lda, B, ldb, ka, kb, a_lane, c_lane, rb + kStep + lrow, r1, ya, yb);
+    tn_mfma<TA, TC, U>(acc, xa, xb);
+    tn_load<TA, TC, U, VEC>(A, lda, B, ldb, ka, kb, a_lane, c_lane, rb + 2 * kStep + lrow, r1, xa, xb);
+    tn_mfma<TA, TC, U>(acc, ya, yb);
+  }
+}
+
+// partial[chunk][a][c] = sum over the chunk's rows of A[i][a] B[i][c]; the four
+// waves' sums are added in wave order.
+template <int TA, int TC>
 __global__ void __launch_bounds__(256) gemm_tn_kernel(const double* __restrict__ A, int64_t lda,
                                                       const double* __restrict__ B, int64_t ldb, int64_t n,
-                                                      int ka, int kb, int64_t rows_per_chunk,
+                                                      int ka, int kb, int64_t rows_per_chunk, int vec_ok,
                                                       double* __restrict__ partial) {
-  __shared__ double As[kTnRows][33];
-  __shared__ double Bs[kTnRows][33];
+  constexpr int U = TA * TC >= 4 ? 2 : 4;  // 4-row steps per register set
+  __shared__ double red[TA * TC * 4 * 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t chunk = blockIdx.x;
-  const int a0 = blockIdx.y * 32;
-  const int t = threadIdx.x;
-  const int oa = t >> 3;        // 0..31 output row (a)
-  const int oc = (t & 7) * 4;   // 4 output columns
-  double acc[4] = {0, 0, 0, 0};
+  const int a0 = blockIdx.y * 16 * TA;
   const int64_t r0 = chunk * rows_per_chunk, r1 = min(n, r0 + rows_per_chunk);
-  // element e of a tile (e = k * 256 + t, k < 8): row e >> 5, column e & 31
-  double va[8], vb[8];
-  auto load = [&](int64_t rb) {
+  f64x4 acc[TA][TC];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int e = k * 256 + t;
-      const int64_t i = rb + (e >> 5);
-      const int cc = e & 31;
-      va[k] = (i < r1 && a0 + cc < ka) ? A[i * lda + a0 + cc] : 0.0;
-      vb[k] = (i < r1 && cc < kb) ? B[i * ldb + cc] : 0.0;
-    }
-  };
-  if (r0 < r1) load(r0);
-  for (int64_t rb = r0; rb < r1; rb += kTnRows) {
-    __syncthreads();
+  for (int t = 0; t < TA; ++t)
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int e = k * 256 + t;
-      As[e >> 5][e & 31] = va[k];
-      Bs[e >> 5][e & 31] = vb[k];
-    }
-    __syncthreads();
-    if (rb + kTnRows < r1) load(rb + kTnRows);
-#pragma unroll 8
-    for (int rr = 0; rr < kTnRows; ++rr) {
-      const double av = As[rr][oa];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] = fma(av, Bs[rr][oc + q], acc[q]);
+    for (int tc = 0; tc < TC; ++tc) acc[t][tc] = f64x4{0.0, 0.0, 0.0, 0.0};
+  const int a_lane = a0 + TA * (lane & 15);
+  const int c_lane = lane & 15;
+  const int lrow = wave * U * 4 + (lane >> 4);
+  if (r0 < r1) {
+    bool vec = false;
+    if constexpr (TA % 2 == 0) vec = vec_ok && a0 + 16 * TA <= ka;
+    if (vec) {
+      if constexpr (TA % 2 == 0)
+        tn_rows<TA, TC, U, true>(A, lda, B, ldb, ka, kb, a_lane, c_lane, r0, r1, lrow, acc);
+    } else {
+      tn_rows<TA, TC, U, false>(A, lda, B, ldb, ka, kb, a_lane, c_lane, r0, r1, lrow, acc);
     }
   }
+  for (int s = 1; s < kTnWaves; ++s) {
+    __syncthreads();
+    if (wave == s) {
+#pragma unroll
+      for (int t = 0; t < TA; ++t)
+#pragma unroll
+        for (int tc = 0; tc < TC; ++tc)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) red[((t * TC + tc) * 4 + r) * 64 + lane] = acc[t][tc][r];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+      for (int t = 0; t < TA; ++t)
+#pragma unroll
+        for (int tc = 0; tc < TC; ++tc)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[t][tc][r] += red[((t * TC + tc) * 4 + r) * 64 + lane];
+    }
+  }
+  if (wave != 0) return;
   double* P = partial + chunk * (int64_t)ka * kb;
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
-    if (a0 + oa < ka && oc + q < kb) P[(int64_t)(a0 + oa) * kb + oc + q] = acc[q];
+  for (int t = 0; t < TA; ++t)
+#pragma unroll
+    for (int tc = 0; tc < TC; ++tc)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = a0 + TA * ((lane >> 4) + 4 * r) + t;
+        const int c = 16 * tc + c_lane;
+        if (a < ka && c < kb) P[(int64_t)a * kb + c] = acc[t][tc][r];
+      }
 }
 
-__global__ void reduce_partials_kernel(const double* partial, int64_t nchunks, int64_t count, double* C) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  double s = 0.0;
-  for (int64_t q = 0; q < nchunks; ++q) s += partial[q * count + i];
-  C[i] = s;
+// C[i] = sum over chunks of partial[chunk][i], chunks in ascending order within
+// 16 segments, the segments then added in order.
+constexpr int kRedSegs = 16;
+__global__ void __launch_bounds__(256) reduce_partials_kernel(const double* __restrict__ partial,
+                                                              int64_t nchunks, int64_t count,
+                                                              double* __restrict__ C) {
+  __shared__ double red[kRedSegs][16];
+  const int o = threadIdx.x & 15, seg = threadIdx.x >> 4;
+  const int64_t i = (int64_t)blockIdx.x * 16 + o;
+  double acc = 0.0;
+  if (i < count) {
+    const int64_t q0 = nchunks * seg / kRedSegs, q1 = nchunks * (seg + 1) / kRedSegs;
+#pragma unroll 8
+    for (int64_t q = q0; q < q1; ++q) acc += partial[q * count + i];
+  }
+  red[seg][o] = acc;
+  __syncthreads();
+  if (seg == 0 && i < count) {
+    double s = red[0][o];
+    for (int k = 1; k < kRedSegs; ++k) s += red[k][o];
+    C[i] = s;
+  }
 }
 
-// C[i][c] = beta*C[i][c] + alpha * sum_a A[i][a] B[a][c]. A block owns 64 rows:
-// 32-column slices of A are staged in LDS with coalesced loads (a wave reads two
-// 256-byte row segments per instruction); thread (row t/4, columns t%4 + 4j)
-// keeps up to 8 accumulators. The sum runs over a in ascending order.
-constexpr int kNnRows = 64;
-__global__ void __launch_bounds__(256) gemm_nn_kernel(const double* __restrict__ A, int64_t lda,
-                                                      const double* __restrict__ B, int64_t n, int ka,
-                                                      int kb, double alpha, double beta,
-                                                      double* __restrict__ C, int64_t ldc) {
-  __shared__ double As[kNnRows][33];
-  __shared__ double Bs[32][33];
-  const int t = threadIdx.x;
-  const int r = t >> 2, cg = t & 3;
-  const int64_t row0 = (int64_t)blockIdx.x * kNnRows;
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  // element e of an A slice (e = k * 256 + t): row e >> 5, column a0 + (e & 31);
-  // the next slice is loaded into registers while the current one is used
-  double va[8];
-  auto load = [&](int a0) {
+// --- C = beta C + alpha A B -------------------------------------------------
+// MFMA M = columns of C (16 per tile, TC tiles), N = rows, K = columns of A.
+// B^T for up to kNnSlab columns of A is staged in LDS once per block. A wave
+// owns 2 tiles of 16 rows; for each 32-column slice of A, lane l loads columns
+// a0 + 8*(l/16) .. +7 of row l%16 of each tile (64 contiguous bytes), and MFMA
+// e (e < 8) stands for column a0 + 8*(l/16) + e in K slot l/16. The sum over A's
+// columns runs slice by slice in a fixed order.
+constexpr int kNnWaves = 4;
+constexpr int kNnTiles = 2;                          // 16-row tiles per wave
+constexpr int kNnRows = kNnWaves * kNnTiles * 16;    // rows per block step
+template <int TC>
+struct NnSlab {
+  static constexpr int cols = TC == 1 ? 384 : 192;   // columns of A per launch
+  static constexpr int zero_slice = cols / 32;       // an all-zero slice after them
+  static constexpr int stride = cols + 32 + 2;       // LDS row stride (doubles)
+};
+
+template <bool VEC>
+__device__ __forceinline__ void nn_load(const double* __restrict__ A, int64_t lda, int ka,
+                                        const int64_t (&rows)[kNnTiles], int col0,
+                                        double (&v)[kNnTiles][8]) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int e = k * 256 + t;
-      const int64_t i = row0 + (e >> 5);
-      va[k] = (i < n && a0 + (e & 31) < ka) ? A[i * lda + a0 + (e & 31)] : 0.0;
-    }
-  };
-  load(0);
-  for (int a0 = 0; a0 < ka; a0 += 32) {
-    __syncthreads();
+  for (int tt = 0; tt < kNnTiles; ++tt) {
+    const double* ap = A + rows[tt] * lda;
+    if constexpr (VEC) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int e = k * 256 + t;
-      As[e >> 5][e & 31] = va[k];
-    }
+      for (int e = 0; e < 8; e += 2) {
+        const double2 x = *reinterpret_cast<const double2*>(ap + col0 + e);
+        v[tt][e] = x.x;
+        v[tt][e + 1] = x.y;
+      }
+    } else {
+      // columns past ka read column ka - 1 and meet zero rows of the staged B^T
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int e = k * 256 + t;
-      const int ra = e >> 5, cc = e & 31;
-      Bs[ra][cc] = (a0 + ra < ka && cc < kb) ? B[(int64_t)(a0 + ra) * kb + cc] : 0.0;
-    }
-    __syncthreads();
-    if (a0 + 32 < ka) load(a0 + 32);
-    const int lim = min(32, ka - a0);
-    for (int ra = 0; ra < lim; ++ra) {
-      const double av = As[r][ra];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) acc[j] = fma(av, Bs[ra][cg + 4 * j], acc[j]);
+      for (int e = 0; e < 8; ++e) v[tt][e] = ap[min(col0 + e, ka - 1)];
     }
   }
-  const int64_t i = row0 + r;
-  if (i < n) {
+}
+
+template <int TC>
+__device__ __forceinline__ void nn_mfma(f64x4 (&acc)[kNnTiles][TC], const double* bt, int lane, int s,
+                                        const double (&v)[kNnTiles][8]) {
+  // B^T values of this lane: row 16*tc + l%16, columns 32*s + 8*(l/16) + e
+  const double* bp = bt + (lane & 15) * NnSlab<TC>::stride + 32 * s + 8 * (lane >> 4);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int c = cg + 4 * j;
-      if (c < kb) {
-        const double prev = beta == 0.0 ? 0.0 : beta * C[i * ldc + c];
-        C[i * ldc + c] = prev + alpha * acc[j];
+  for (int e = 0; e < 8; e += 2) {
+#pragma unroll
+    for (int tc = 0; tc < TC; ++tc) {
+      const double2 b = *reinterpret_cast<const double2*>(bp + 16 * tc * NnSlab<TC>::stride + e);
+#pragma unroll
+      for (int tt = 0; tt < kNnTiles; ++tt) {
+        acc[tt][tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(b.x, v[tt][e], acc[tt][tc], 0, 0, 0);
+        acc[tt][tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(b.y, v[tt][e + 1], acc[tt][tc], 0, 0, 0);
       }
+    }
+  }
+}
+
+// Slices [s0, s1) of 32 columns; two register sets, each loaded one slice ahead.
+// An odd count ends with a pair whose second half meets the zero slice of B^T,
+// so the loop body has no branch between a load and its use.
+template <int TC, bool VEC>
+__device__ __forceinline__ void nn_slices(const double* __restrict__ A, int64_t lda, int ka,
+                                          const int64_t (&rows)[kNnTiles], const double* bt, int lane,
+                                          int s0, int s1, f64x4 (&acc)[kNnTiles][TC]) {
+  if (s0 >= s1) return;
+  const int kofs = 8 * (lane >> 4);
+  double x[kNnTiles][8], y[kNnTiles][8];
+  nn_load<VEC>(A, lda, ka, rows, 32 * s0 + kofs, x);
+  for (int s = s0; s < s1; s += 2) {
+    nn_load<VEC>(A, lda, ka, rows, 32 * min(s + 1, s1 - 1) + kofs, y);
+    nn_mfma<TC>(acc, bt, lane, s, x);
+    nn_load<VEC>(A, lda, ka, rows, 32 * min(s + 2, s1 - 1) + kofs, x);
+    nn_mfma<TC>(acc, bt, lane, s + 1 < s1 ? s + 1 : NnSlab<TC>::zero_slice, y);
+  }
+}
+
+template <int TC>
+__global__ void __launch_bounds__(256) gemm_nn_kernel(const double* __restrict__ A, int64_t lda,
+                                                      const double* __restrict__ B, int64_t n, int ka,
+                                                      int kb, double alpha, double beta, int vec_ok,
+                                                      double* __restrict__ C, int64_t ldc) {
+  constexpr int S = NnSlab<TC>::stride;
+  __shared__ __align__(16) double bt[16 * TC * S];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // B^T, zero-padded to 16*TC rows and whole 32-column slices, then the zero slice
+  const int nsl = (ka + 31) / 32;
+  for (int e = threadIdx.x; e < 16 * TC * nsl * 32; e += 256) {
+    const int a = e / (16 * TC), c = e % (16 * TC);
+    bt[c * S + a] = (a < ka && c < kb) ? B[(int64_t)a * kb + c] : 0.0;
+  }
+  for (int e = threadIdx.x; e < 16 * TC * 32; e += 256)
+    bt[(e >> 5) * S + 32 * NnSlab<TC>::zero_slice + (e & 31)] = 0.0;
+  __syncthreads();
+  const int nfull = vec_ok ? ka / 32 : 0;
+  for (int64_t rb = (int64_t)blockIdx.x * kNnRows; rb < n; rb += (int64_t)gridDim.x * kNnRows) {
+    int64_t rows[kNnTiles];
+#pragma unroll
+    for (int tt = 0; tt < kNnTiles; ++tt)
+      rows[tt] = min(n - 1, rb + (wave * kNnTiles + tt) * 16 + (lane & 15));
+    f64x4 acc[kNnTiles][TC];
+#pragma unroll
+    for (int tt = 0; tt < kNnTiles; ++tt)
+#pragma unroll
+      for (int tc = 0; tc < TC; ++tc) acc[tt][tc] = f64x4{0.0, 0.0, 0.0, 0.0};
+    nn_slices<TC, true>(A, lda, ka, rows, bt, lane, 0, nfull, acc);
+    nn_slices<TC, false>(A, lda, ka, rows, bt, lane, nfull, nsl, acc);
+#pragma unroll
+    for (int tt = 0; tt < kNnTiles; ++tt) {
+      const int64_t i = rb + (wave * kNnTiles + tt) * 16 + (lane & 15);
+      if (i >= n) continue;
+#pragma unroll
+      for (int tc = 0; tc < TC; ++tc)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int c = 16 * tc + (lane >> 4) + 4 * r;
+          if (c < kb) {
+            const double prev = beta == 0.0 ? 0.0 : beta * C[i * ldc + c];
+            C[i * ldc + c] = prev + alpha * acc[tt][tc][r];
+          }
+        }
     }
   }
 }
@@ -883,30 +1054,71 @@ int snapgpu_gemm_tn_f64(const double* d_A, int64_t lda, const double* d_B, int64
                         void* stream) {
   clear_error();
   SG_ARG(ka >= 1 && kb >= 1 && kb <= 32, "need 1 <= kb <= 32");
+  SG_ARG(n >= 0 && lda >= ka && ldb >= kb, "need n >= 0, lda >= ka, ldb >= kb");
   SG_ARG(ws_bytes >= snapgpu_gemm_tn_workspace_bytes(n, ka, kb), "workspace too small");
   hipStream_t st = as_stream(stream);
   const int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(512, ceil_div(n, 4096)));
   const int64_t rpc = ceil_div(std::max<int64_t>(n, 1), chunks);
   double* part = (double*)d_ws;
-  gemm_tn_kernel<<<dim3((unsigned)chunks, (unsigned)ceil_div(ka, 32)), 256, 0, st>>>(
-      d_A, lda, d_B, ldb, n, ka, kb, rpc, part);
+  // 16*TA columns of A per block: as few column groups as 128-wide ones need,
+  // each as narrow as covers ka
+  const int groups = (int)ceil_div(ka, 128);
+  int ta = (int)ceil_div(ka, 16 * groups);
+  if (ta > 1 && (ta & 1)) ++ta;
+  const int tc = kb <= 16 ? 1 : 2;
+  const int vec_ok = (lda % 2 == 0) && (((uintptr_t)d_A & 15) == 0);
+  const dim3 grid((unsigned)chunks, (unsigned)ceil_div(ka, 16 * ta));
+#define SG_TN(TA, TC)                                                                              \
+  gemm_tn_kernel<TA, TC><<<grid, 256, 0, st>>>(d_A, lda, d_B, ldb, n, ka, kb, rpc, vec_ok, part)
+#define SG_TN_TA(TC)      \
+  switch (ta) {           \
+    case 1: SG_TN(1, TC); break; \
+    case 2: SG_TN(2, TC); break; \
+    case 4: SG_TN(4, TC); break; \
+    case 6: SG_TN(6, TC); break; \
+    default: SG_TN(8, TC); break; \
+  }
+  if (tc == 1) {
+    SG_TN_TA(1)
+  } else {
+    SG_TN_TA(2)
+  }
+#undef SG_TN_TA
+#undef SG_TN
   SG_LAUNCH_CHECK();
   const int64_t count = (int64_t)ka * kb;
-  reduce_partials_kernel<<<(unsigned)ceil_div(count, 256), 256, 0, st>>>(part, chunks, count, d_C);
+  reduce_partials_kernel<<<(unsigned)ceil_div(count, 16), 256, 0, st>>>(part, chunks, count, d_C);
   SG_LAUNCH_CHECK();
   return SNAPGPU_OK;
 }
 
 // C[n x kb] = beta*C + alpha * A[n x ka] B[ka x kb] (B row-major, contiguous).
+// Columns of A beyond one LDS slab of B^T go through further launches that add
+// to C.
 int snapgpu_gemm_nn_f64(const double* d_A, int64_t lda, const double* d_B, int64_t n, int32_t ka,
                         int32_t kb, double alpha, double beta, double* d_C, int64_t ldc,
                         void* stream) {
   clear_error();
   SG_ARG(ka >= 1 && kb >= 1 && kb <= 32, "need 1 <= kb <= 32");
+  SG_ARG(lda >= ka && ldc >= kb, "need lda >= ka, ldc >= kb");
   if (n == 0) return SNAPGPU_OK;
-  gemm_nn_kernel<<<(unsigned)ceil_div(n, kNnRows), 256, 0, as_stream(stream)>>>(d_A, lda, d_B, n, ka,
-                                                                                kb, alpha, beta, d_C, ldc);
-  SG_LAUNCH_CHECK();
+  hipStream_t st = as_stream(stream);
+  const int tc = kb <= 16 ? 1 : 2;
+  const int slab = tc == 1 ? NnSlab<1>::cols : NnSlab<2>::cols;
+  const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(n, kNnRows), 4096);
+  for (int a0 = 0; a0 < ka; a0 += slab) {
+    const int w = std::min(slab, ka - a0);
+    const double* A = d_A + a0;
+    const int vec_ok = (lda % 2 == 0) && (((uintptr_t)A & 15) == 0);
+    const double bt = a0 == 0 ? beta : 1.0;
+    if (tc == 1)
+      gemm_nn_kernel<1><<<grid, 256, 0, st>>>(A, lda, d_B + (int64_t)a0 * kb, n, w, kb, alpha, bt,
+                                               vec_ok, d_C, ldc);
+    else
+      gemm_nn_kernel<2><<<grid, 256, 0, st>>>(A, lda, d_B + (int64_t)a0 * kb, n, w, kb, alpha, bt,
+                                               vec_ok, d_C, ldc);
+    SG_LAUNCH_CHECK();
+  }
   return SNAPGPU_OK;
 }
 
