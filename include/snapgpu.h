@@ -123,6 +123,18 @@ int snapgpu_tile_fill(const uint64_t* d_cell_ptr, const uint32_t* d_pos,
 int snapgpu_csr_col_sum_u32(const void* d_indices, int32_t index_bits, const uint32_t* d_values,
                             int64_t nnz, uint64_t* d_out, void* stream);
 
+/* d_out[j] = sum of the stored counts of column j (uint64; written, not added
+ * to) for a CSR matrix given by its row pointers. Same sums as
+ * snapgpu_csr_col_sum_u32 over the entries (select_features' column sums,
+ * snapatac2-python/python/snapatac2/preprocessing/_basic.py:1053-1055), with
+ * one global atomic per (block of 3968 rows, non-zero column) instead of one
+ * per entry when every row's column indices ascend; rows out of order are
+ * detected and summed entry by entry instead. Indices must lie in
+ * [0, n_cols). d_ws16: 16 bytes of device scratch. Synchronises `stream`. */
+int snapgpu_csr_col_sum_rows_u32(const int64_t* d_indptr, int64_t n_rows, const void* d_indices,
+                                 int32_t index_bits, const uint32_t* d_values, int64_t n_cols,
+                                 uint64_t* d_out, void* d_ws16, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Synthetic workload generator (no reference counterpart: it produces the   */
 /* obsm['fragment_paired'] encoding of import.rs:161-210 from a seed).       */

@@ -43,6 +43,13 @@ void clear_error();
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Column counts (entries per column) of a CSR matrix into d_out (zeroed by the
+// caller) when every row's column indices ascend; sets *sorted = false, with
+// d_out then incomplete, otherwise. d_status: 4 bytes of device scratch.
+// Synchronises `st`.
+int col_counts_sorted(const int64_t* d_indptr, int64_t n, const int32_t* d_idx, int64_t nnz, int64_t m,
+                      unsigned long long* d_out, int32_t* d_status, hipStream_t st, bool* sorted);
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 constexpr int kWave = 64;

@@ -849,7 +849,7 @@ int64_t snapgpu_csr_transpose_workspace_bytes(int64_t m, int64_t chunk_nnz) {
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                                            (const uint32_t*)nullptr, (uint32_t*)nullptr,
                                            (int)std::min<int64_t>(chunk_nnz, INT32_MAX));
-  return (int64_t)m * 8 + 256 + chunk_nnz * 4 * 4 + 4 * 256 + (int64_t)tmp + 256;
+  return (int64_t)m * 8 + 256 + chunk_nnz * 4 * 4 + 5 * 256 + (int64_t)tmp + 256;
 }
 
 // Stable transpose. colptr[m+1] out; rows ascending inside each column.
@@ -873,13 +873,22 @@ int snapgpu_csr_transpose(const int64_t* d_indptr, const int32_t* d_idx, const u
   uint32_t* keys_out = (uint32_t*)take(chunk_nnz * 4);
   uint32_t* pos_in = (uint32_t*)take(chunk_nnz * 4);
   uint32_t* pos_out = (uint32_t*)take(chunk_nnz * 4);
+  int32_t* wstatus = (int32_t*)take(4);
   void* tmp = p;
   size_t tmp_bytes = (size_t)(ws_bytes - (p - (char*)d_ws));
   // column counts -> colptr
   SG_HIP(hipMemsetAsync(d_colptr, 0, (m + 1) * 8, st));
   if (nnz > 0) {
-    col_count_kernel<<<grid_for(nnz, 256, 16384), 256, 0, st>>>(d_idx, nnz, (unsigned long long*)(d_colptr + 1));
-    SG_LAUNCH_CHECK();
+    bool sorted = true;
+    int rc = col_counts_sorted(d_indptr, n, d_idx, nnz, m, (unsigned long long*)(d_colptr + 1), wstatus,
+                               st, &sorted);
+    if (rc) return rc;
+    if (!sorted) {  // rows out of order: one atomic per entry
+      SG_HIP(hipMemsetAsync(d_colptr, 0, (m + 1) * 8, st));
+      col_count_kernel<<<grid_for(nnz, 256, 16384), 256, 0, st>>>(d_idx, nnz,
+                                                                  (unsigned long long*)(d_colptr + 1));
+      SG_LAUNCH_CHECK();
+    }
   }
   const int64_t sws = snapgpu_scan_workspace_bytes(m);
   SG_ARG((int64_t)tmp_bytes >= sws, "workspace too small for the scan");

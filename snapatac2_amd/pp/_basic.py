@@ -300,10 +300,12 @@ def _column_sums(X) -> np.ndarray:
         X = DeviceCSR(torch.from_numpy(Xc.indptr.astype(np.int64)).to(dev),
                       torch.from_numpy(Xc.indices.astype(np.int32 if Xc.shape[1] < 2**31 else np.int64)).to(dev),
                       torch.from_numpy(Xc.data.astype(np.uint32).view(np.int32)).to(dev), Xc.shape)
-    out = torch.zeros(X.shape[1], dtype=torch.int64, device=X.device)
+    out = torch.empty(X.shape[1], dtype=torch.int64, device=X.device)
+    ws = torch.empty(4, dtype=torch.int32, device=X.device)
     bits = 32 if X.indices.dtype == torch.int32 else 64
-    _lib.check(L.snapgpu_csr_col_sum_u32(X.indices.data_ptr(), bits, X.data.data_ptr(), X.nnz,
-                                         out.data_ptr(), _lib.stream_ptr()))
+    _lib.check(L.snapgpu_csr_col_sum_rows_u32(X.indptr.data_ptr(), X.shape[0], X.indices.data_ptr(), bits,
+                                              X.data.data_ptr(), X.shape[1], out.data_ptr(),
+                                              ws.data_ptr(), _lib.stream_ptr()))
     return out.cpu().numpy().astype(np.float64)
 
 
