@@ -196,14 +196,23 @@ int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, 
 /* ------------------------------------------------------------------------ */
 
 /* Column selection by feature mask (spectral_embedding's slice_axis,
- * embedding.rs:36-41). d_col_map[old column] = new column or -1. */
+ * embedding.rs:36-41): the selected columns keep their order.
+ * snapgpu_col_select_table: d_cols = the k selected columns, ascending and
+ * distinct, in [0, m); d_sel = 2 * ceil(m / 32) words: for word w, the mask of
+ * selected columns among 32w .. 32w + 31 and the count of selected columns
+ * below 32w. Entries with columns outside [0, m) are dropped.
+ * select_count: d_row_nnz[i] = selected entries of row i.
+ * select_fill: copies them, columns renumbered 0 .. k-1, into the CSR given by
+ * d_new_indptr (the scan of the counts). */
+int snapgpu_col_select_table(const int64_t* d_cols, int64_t k, int64_t m, uint32_t* d_sel,
+                             void* stream);
 int snapgpu_csr_select_count(const int64_t* d_indptr, const void* d_idx, int32_t index_bits,
-                             int64_t n, const int32_t* d_col_map, int64_t* d_row_nnz,
+                             int64_t n, const uint32_t* d_sel, int64_t m, int64_t* d_row_nnz,
                              void* stream);
 int snapgpu_csr_select_fill(const int64_t* d_indptr, const void* d_idx, int32_t index_bits,
-                            const uint32_t* d_val, int64_t n, const int32_t* d_col_map,
-                            const int64_t* d_new_indptr, int32_t* d_new_idx,
-                            uint32_t* d_new_val, void* stream);
+                            const uint32_t* d_val, int64_t n, const uint32_t* d_sel, int64_t m,
+                            const int64_t* d_new_indptr, int32_t* d_new_idx, uint32_t* d_new_val,
+                            void* stream);
 
 /* Stable CSR -> CSC transpose (rows ascending within each column, so column
  * reductions add in the reference's CSR order, embedding.rs:139-144).

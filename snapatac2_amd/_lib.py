@@ -38,8 +38,9 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_knn_workspace_bytes": (_i64, [_i64, _i32, _i32]),
     "snapgpu_knn": (C.c_int, [_p, _i64, _i32, _i64, _i32, _i32, _p, _p, _p, _i64, _i32,
                               C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64), _p]),
-    "snapgpu_csr_select_count": (C.c_int, [_p, _p, _i32, _i64, _p, _p, _p]),
-    "snapgpu_csr_select_fill": (C.c_int, [_p, _p, _i32, _p, _i64, _p, _p, _p, _p, _p]),
+    "snapgpu_col_select_table": (C.c_int, [_p, _i64, _i64, _p, _p]),
+    "snapgpu_csr_select_count": (C.c_int, [_p, _p, _i32, _i64, _p, _i64, _p, _p]),
+    "snapgpu_csr_select_fill": (C.c_int, [_p, _p, _i32, _p, _i64, _p, _i64, _p, _p, _p, _p]),
     "snapgpu_csr_transpose_workspace_bytes": (_i64, [_i64, _i64]),
     "snapgpu_csr_transpose": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _p, _i64, _p, _p, _p, _p,
                                         _i64, _p]),

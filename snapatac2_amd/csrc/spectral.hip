@@ -29,15 +29,52 @@ namespace spectral {
 // ---------------------------------------------------------------------------
 // Column selection of a CSR count matrix (feature mask -> new column ids)
 // ---------------------------------------------------------------------------
+// The selection is a table of 2 words per 32 columns, sel[2w] = bit mask of the
+// selected columns among 32w .. 32w + 31 and sel[2w + 1] = number of selected
+// columns below 32w: 8 bytes per 32 columns (1.5 MB for 6.2M tiles) stays in
+// L2, where a 4-byte-per-column map would not. New id of a selected column c =
+// sel[2w + 1] + popcount(sel[2w] & bits below c % 32), i.e. its rank.
+__global__ void col_select_table_kernel(const int64_t* __restrict__ cols, int64_t k, int64_t words,
+                                        uint32_t* __restrict__ sel) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= words) return;
+  auto lower = [&](int64_t v) {  // first i with cols[i] >= v
+    int64_t lo = 0, hi = k;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (cols[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+  };
+  const int64_t a = lower(32 * w), b = lower(32 * w + 32);
+  uint32_t mask = 0;
+  for (int64_t i = a; i < b; ++i) {
+    const int64_t d = cols[i] - 32 * w;
+    if (d >= 0 && d < 32) mask |= 1u << d;
+  }
+  sel[2 * w] = mask;
+  sel[2 * w + 1] = (uint32_t)a;
+}
+
+template <typename IdxT>
+__device__ __forceinline__ uint2 sel_word(const uint32_t* __restrict__ sel, int64_t words, IdxT c) {
+  const int64_t w = (int64_t)c >> 5;
+  if (c < 0 || w >= words) return make_uint2(0u, 0u);
+  return *reinterpret_cast<const uint2*>(sel + 2 * w);
+}
+
 template <typename IdxT>
 __global__ void __launch_bounds__(256) select_count_kernel(const int64_t* indptr, const IdxT* idx,
-                                                           int64_t n, const int32_t* col_map,
+                                                           int64_t n, const uint32_t* sel, int64_t words,
                                                            int64_t* row_nnz) {
   const int lane = threadIdx.x & 63;
   const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (row >= n) return;
   int64_t cnt = 0;
-  for (int64_t e = indptr[row] + lane; e < indptr[row + 1]; e += 64) cnt += col_map[idx[e]] >= 0;
+  for (int64_t e = indptr[row] + lane; e < indptr[row + 1]; e += 64) {
+    const IdxT c = idx[e];
+    cnt += (sel_word(sel, words, c).x >> ((uint32_t)c & 31u)) & 1u;
+  }
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
   if (lane == 0) row_nnz[row] = cnt;
 }
@@ -45,7 +82,7 @@ __global__ void __launch_bounds__(256) select_count_kernel(const int64_t* indptr
 template <typename IdxT>
 __global__ void __launch_bounds__(256) select_fill_kernel(const int64_t* indptr, const IdxT* idx,
                                                           const uint32_t* val, int64_t n,
-                                                          const int32_t* col_map,
+                                                          const uint32_t* sel, int64_t words,
                                                           const int64_t* new_indptr,
                                                           int32_t* new_idx, uint32_t* new_val) {
   const int lane = threadIdx.x & 63;
@@ -56,7 +93,12 @@ __global__ void __launch_bounds__(256) select_fill_kernel(const int64_t* indptr,
   for (int64_t e0 = indptr[row]; e0 < e1; e0 += 64) {
     const int64_t e = e0 + lane;
     int32_t nc = -1;
-    if (e < e1) nc = col_map[idx[e]];
+    if (e < e1) {
+      const IdxT c = idx[e];
+      const uint2 s = sel_word(sel, words, c);
+      const uint32_t bit = (uint32_t)c & 31u;
+      if ((s.x >> bit) & 1u) nc = (int32_t)(s.y + __popc(s.x & ((1u << bit) - 1u)));
+    }
     const uint64_t m = ballot64(nc >= 0);
     if (nc >= 0) {
       const int64_t o = out + popc64(m & lanemask_lt(lane));
@@ -91,39 +133,36 @@ __global__ void iota_kernel(uint32_t* v, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) v[i] = (uint32_t)i;
 }
 
-// Places the sorted chunk: position = cursor[col] + rank of the element within
-// its column segment of this chunk.
+// Runs of equal keys in a sorted chunk: cursor[col] is the next free CSC slot of
+// column col. rebase: cursor[col] -= t0 at the run's first position t0, so the
+// slot of sorted position t is cursor[key[t]] + t; advance: cursor[col] += t + 1
+// at the run's last position (the column's next free slot again). One thread
+// touches each column present, so no atomics.
+__global__ void cursor_rebase_kernel(const uint32_t* keys, int64_t cn, int64_t* cursor) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cn; t += stride) {
+    const uint32_t col = keys[t];
+    if (t == 0 || keys[t - 1] != col) cursor[col] -= t;
+  }
+}
+
 __global__ void scatter_chunk_kernel(const uint32_t* skeys, const uint32_t* spos, int64_t cn,
                                      const uint32_t* row_of, const uint32_t* cval,
                                      const int64_t* cursor, int32_t* rowidx, uint32_t* out_val) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cn; t += stride) {
-    const uint32_t col = skeys[t];
-    int64_t lo = 0, hi = t;  // first position with key == col
-    while (lo < hi) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (skeys[mid] < col) lo = mid + 1; else hi = mid;
-    }
-    const int64_t o = cursor[col] + (t - lo);
+    const int64_t o = cursor[skeys[t]] + t;
     const uint32_t p = spos[t];
     rowidx[o] = (int32_t)row_of[p];
     out_val[o] = cval[p];
   }
 }
 
-// cursor[col] += entries of col in this chunk: the last element of each run of
-// equal keys adds the run's length (one atomic per column present).
 __global__ void cursor_advance_kernel(const uint32_t* keys, int64_t cn, int64_t* cursor) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cn; t += stride) {
     const uint32_t col = keys[t];
-    if (t + 1 < cn && keys[t + 1] == col) continue;
-    int64_t lo = 0, hi = t;  // first position with key == col
-    while (lo < hi) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (keys[mid] < col) lo = mid + 1; else hi = mid;
-    }
-    atomicAdd(reinterpret_cast<unsigned long long*>(&cursor[col]), (unsigned long long)(t - lo + 1));
+    if (t + 1 == cn || keys[t + 1] != col) cursor[col] += t + 1;
   }
 }
 
@@ -661,6 +700,21 @@ __global__ void scale_rows_f32_kernel(float* A, int64_t rows, int BW, const floa
   A[t] *= sc[t / BW];
 }
 
+// [r o V | r o G | 0] in BW-wide FP32 rows (2b <= BW): both operands of the
+// adjoint's sparse products in one pass each
+template <int BW>
+__global__ void scale_in_pair_kernel(const double* V, int64_t ldv, const double* G, int64_t ldg, int64_t n,
+                                     int b, const float* r, float* out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t / BW;
+  const int c = (int)(t % BW);
+  if (i >= n) return;
+  float v = 0.f;
+  if (c < b) v = (float)((double)r[i] * V[i * ldv + c]);
+  else if (c < 2 * b) v = (float)((double)r[i] * G[i * ldg + c - b]);
+  out[i * BW + c] = v;
+}
+
 // alpha, gamma and t = gamma u per cell (all zero for an empty row)
 __global__ void adj_rows_kernel(const double* G, int64_t ldg, const double* V, int64_t ldv,
                                 const float* XP, const float* XQ, int BW, int b, int64_t n,
@@ -777,23 +831,38 @@ static void launch_vjp(const int64_t* indptr, const int32_t* idx, const uint32_t
   double* phi = (double*)take(m * 8);
   double* sigma = (double*)take(m * 8);
   const unsigned gn = (unsigned)ceil_div(n, 4), gm = (unsigned)ceil_div(m, 4);
-  scale_in_kernel<BW><<<(unsigned)ceil_div(n * BW, 256), 256, 0, st>>>(V, ldv, n, b, r32, Vs);
-  scale_in_kernel<BW><<<(unsigned)ceil_div(n * BW, 256), 256, 0, st>>>(G, ldg, n, b, r32, Gs);
-  // P = X^T (r o V), Q = X^T (r o G)
-  spmm_kernel<BW, 0, FV><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, Vs, nullptr, Pm, nullptr, 0,
-                                             nullptr, nullptr, 0, b);
-  spmm_kernel<BW, 0, FV><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, Gs, nullptr, Qm, nullptr, 0,
-                                             nullptr, nullptr, 0, b);
-  rowdot_f32_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(Pm, Qm, m, BW, b, h);
-  scale_rows_f32_kernel<<<(unsigned)ceil_div(m * BW, 256), 256, 0, st>>>(Pm, m, BW, w2);
-  scale_rows_f32_kernel<<<(unsigned)ceil_div(m * BW, 256), 256, 0, st>>>(Qm, m, BW, w2);
-  // X (w^2 o P) into Vs and X (w^2 o Q) into Gs (both free after the CSC pass)
-  spmm_kernel<BW, 0, FV><<<gn, 256, 0, st>>>(indptr, idx, cnt, n, Pm, nullptr, Vs, nullptr, 0,
-                                             nullptr, nullptr, 0, b);
-  spmm_kernel<BW, 0, FV><<<gn, 256, 0, st>>>(indptr, idx, cnt, n, Qm, nullptr, Gs, nullptr, 0,
-                                             nullptr, nullptr, 0, b);
-  adj_rows_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(G, ldg, V, ldv, Vs, Gs, BW, b, n, u, dinv,
-                                                              alpha, gamma, t);
+  if (2 * b <= BW) {
+    // V and G side by side: one CSC and one CSR pass
+    scale_in_pair_kernel<BW><<<(unsigned)ceil_div(n * BW, 256), 256, 0, st>>>(V, ldv, G, ldg, n, b, r32, Vs);
+    // [P | Q] = X^T (r o [V | G])
+    spmm_kernel<BW, 0, FV><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, Vs, nullptr, Pm, nullptr, 0,
+                                               nullptr, nullptr, 0, 2 * b);
+    rowdot_f32_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(Pm, Pm + b, m, BW, b, h);
+    scale_rows_f32_kernel<<<(unsigned)ceil_div(m * BW, 256), 256, 0, st>>>(Pm, m, BW, w2);
+    // X (w^2 o [P | Q]) into Vs (free after the CSC pass)
+    spmm_kernel<BW, 0, FV><<<gn, 256, 0, st>>>(indptr, idx, cnt, n, Pm, nullptr, Vs, nullptr, 0,
+                                               nullptr, nullptr, 0, 2 * b);
+    adj_rows_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(G, ldg, V, ldv, Vs, Vs + b, BW, b, n, u,
+                                                                dinv, alpha, gamma, t);
+  } else {
+    scale_in_kernel<BW><<<(unsigned)ceil_div(n * BW, 256), 256, 0, st>>>(V, ldv, n, b, r32, Vs);
+    scale_in_kernel<BW><<<(unsigned)ceil_div(n * BW, 256), 256, 0, st>>>(G, ldg, n, b, r32, Gs);
+    // P = X^T (r o V), Q = X^T (r o G)
+    spmm_kernel<BW, 0, FV><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, Vs, nullptr, Pm, nullptr, 0,
+                                               nullptr, nullptr, 0, b);
+    spmm_kernel<BW, 0, FV><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, Gs, nullptr, Qm, nullptr, 0,
+                                               nullptr, nullptr, 0, b);
+    rowdot_f32_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(Pm, Qm, m, BW, b, h);
+    scale_rows_f32_kernel<<<(unsigned)ceil_div(m * BW, 256), 256, 0, st>>>(Pm, m, BW, w2);
+    scale_rows_f32_kernel<<<(unsigned)ceil_div(m * BW, 256), 256, 0, st>>>(Qm, m, BW, w2);
+    // X (w^2 o P) into Vs and X (w^2 o Q) into Gs (both free after the CSC pass)
+    spmm_kernel<BW, 0, FV><<<gn, 256, 0, st>>>(indptr, idx, cnt, n, Pm, nullptr, Vs, nullptr, 0,
+                                               nullptr, nullptr, 0, b);
+    spmm_kernel<BW, 0, FV><<<gn, 256, 0, st>>>(indptr, idx, cnt, n, Qm, nullptr, Gs, nullptr, 0,
+                                               nullptr, nullptr, 0, b);
+    adj_rows_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(G, ldg, V, ldv, Vs, Gs, BW, b, n, u, dinv,
+                                                                alpha, gamma, t);
+  }
   // phi = X^T (gamma o u) and e = X^T u in one pass over the CSC
   csc_spmv2_kernel<FV, 1><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, t, u, phi, e);
   csr_spmv_w2_kernel<FV><<<gn, 256, 0, st>>>(indptr, idx, cnt, n, w, phi, psi);
@@ -814,32 +883,52 @@ static inline unsigned grid_for(int64_t work, int per_block, int64_t cap = 1 << 
 
 extern "C" {
 
+int snapgpu_col_select_table(const int64_t* d_cols, int64_t k, int64_t m, uint32_t* d_sel, void* stream) {
+  clear_error();
+  SG_ARG(k >= 0 && m >= 1 && k <= m && k < (1ll << 31), "bad selection size");
+  SG_ARG(d_sel && (k == 0 || d_cols), "null pointer");
+  const int64_t words = ceil_div(m, 32);
+  col_select_table_kernel<<<(unsigned)ceil_div(words, 256), 256, 0, as_stream(stream)>>>(d_cols, k, words,
+                                                                                         d_sel);
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
 int snapgpu_csr_select_count(const int64_t* d_indptr, const void* d_idx, int32_t index_bits,
-                             int64_t n, const int32_t* d_col_map, int64_t* d_row_nnz, void* stream) {
+                             int64_t n, const uint32_t* d_sel, int64_t m, int64_t* d_row_nnz,
+                             void* stream) {
   clear_error();
   SG_ARG(index_bits == 32 || index_bits == 64, "index_bits must be 32 or 64");
   if (n == 0) return SNAPGPU_OK;
   const unsigned g = (unsigned)ceil_div(n, 4);
+  const int64_t words = ceil_div(m, 32);
   if (index_bits == 32)
-    select_count_kernel<int32_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int32_t*)d_idx, n, d_col_map, d_row_nnz);
+    select_count_kernel<int32_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int32_t*)d_idx, n, d_sel,
+                                                                   words, d_row_nnz);
   else
-    select_count_kernel<int64_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int64_t*)d_idx, n, d_col_map, d_row_nnz);
+    select_count_kernel<int64_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int64_t*)d_idx, n, d_sel,
+                                                                   words, d_row_nnz);
   SG_LAUNCH_CHECK();
   return SNAPGPU_OK;
 }
 
 int snapgpu_csr_select_fill(const int64_t* d_indptr, const void* d_idx, int32_t index_bits,
-                            const uint32_t* d_val, int64_t n, const int32_t* d_col_map,
+                            const uint32_t* d_val, int64_t n, const uint32_t* d_sel, int64_t m,
                             const int64_t* d_new_indptr, int32_t* d_new_idx, uint32_t* d_new_val,
                             void* stream) {
   clear_error();
   SG_ARG(index_bits == 32 || index_bits == 64, "index_bits must be 32 or 64");
   if (n == 0) return SNAPGPU_OK;
   const unsigned g = (unsigned)ceil_div(n, 4);
+  const int64_t words = ceil_div(m, 32);
   if (index_bits == 32)
-    select_fill_kernel<int32_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int32_t*)d_idx, d_val, n, d_col_map, d_new_indptr, d_new_idx, d_new_val);
+    select_fill_kernel<int32_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int32_t*)d_idx, d_val, n,
+                                                                  d_sel, words, d_new_indptr, d_new_idx,
+                                                                  d_new_val);
   else
-    select_fill_kernel<int64_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int64_t*)d_idx, d_val, n, d_col_map, d_new_indptr, d_new_idx, d_new_val);
+    select_fill_kernel<int64_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int64_t*)d_idx, d_val, n,
+                                                                  d_sel, words, d_new_indptr, d_new_idx,
+                                                                  d_new_val);
   SG_LAUNCH_CHECK();
   return SNAPGPU_OK;
 }
@@ -919,6 +1008,8 @@ int snapgpu_csr_transpose(const int64_t* d_indptr, const int32_t* d_idx, const u
                                                 (int)cn, 0, end_bit, st));
       // keys_in is free again: reuse it for the row ids of the chunk's entries
       entry_rows_kernel<<<(unsigned)ceil_div(r1 - r0, 4), 256, 0, st>>>(d_indptr, r0, r1, keys_in);
+      SG_LAUNCH_CHECK();
+      cursor_rebase_kernel<<<grid_for(cn, 256, 16384), 256, 0, st>>>(keys_out, cn, cursor);
       SG_LAUNCH_CHECK();
       scatter_chunk_kernel<<<grid_for(cn, 256, 16384), 256, 0, st>>>(keys_out, pos_out, cn, keys_in,
                                                                      d_val + e0, cursor, d_rowidx, d_cval);
@@ -1040,9 +1131,11 @@ int snapgpu_spectral_vjp_w(const int64_t* d_indptr, const int32_t* d_idx, const 
                                d_w, d_r32, d_w2, d_V, ldv, d_G, ldg, b, d_dw, d_ws, st);            \
   else launch_vjp<BW, false>(d_indptr, d_idx, d_cnt, n, d_colptr, d_rowidx, d_ccnt, m, d_s, d_dinv,   \
                              d_w, d_r32, d_w2, d_V, ldv, d_G, ldg, b, d_dw, d_ws, st);
-  if (b <= 8) {
+  // width of the FP32 operand rows: V and G side by side while 2b fits 32
+  const int need = b <= 16 ? 2 * b : b;
+  if (need <= 8) {
     SG_VJP(8)
-  } else if (b <= 16) {
+  } else if (need <= 16) {
     SG_VJP(16)
   } else {
     SG_VJP(32)

@@ -51,8 +51,17 @@ def nearest_neighbour_graph(data, k: int, precision: str = "bf16", force_fallbac
     _lib.check(L.snapgpu_knn(X.data_ptr(), n, d, d, kk, _MODES[precision], nbr.data_ptr(),
                              dist.data_ptr(), ws.data_ptr(), ws_b, flags, C.byref(nfb), C.byref(nex),
                              C.byref(nvis), _lib.stream_ptr()))
-    indptr = np.arange(0, n * kk + 1, kk, dtype=np.int64)
-    out = csr_matrix((dist.cpu().numpy().ravel(), nbr.cpu().numpy().ravel(), indptr), shape=(n, n))
+    # page-locked host copies (the returned arrays keep them alive): one DMA each
+    # instead of staging through pageable memory
+    hd = torch.empty((n, kk), dtype=torch.float64, pin_memory=True)
+    hn = torch.empty((n, kk), dtype=torch.int32, pin_memory=True)
+    hd.copy_(dist, non_blocking=True)
+    hn.copy_(nbr, non_blocking=True)
+    torch.cuda.current_stream().synchronize()
+    ip_t = np.int32 if n * kk < 2**31 else np.int64
+    indptr = np.arange(0, n * kk + 1, kk, dtype=ip_t)
+    out = csr_matrix((hd.numpy().ravel(), hn.numpy().ravel().astype(ip_t, copy=False), indptr),
+                     shape=(n, n), copy=False)
     nwg = -(-n // 128)
     stats = {"fallback_rows": int(nfb.value), "exact_scan_rows": int(nex.value),
              "chunk_visits": int(nvis.value), "chunk_pairs": nwg * nwg if precision == "bf16" else 0}

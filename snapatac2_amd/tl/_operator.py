@@ -46,14 +46,17 @@ def select_columns(X: DeviceCSR, cols: np.ndarray | None) -> DeviceCSR:
         cols = np.arange(X.shape[1])
     L = _lib.lib()
     n, m_all = X.shape
-    col_map = np.full(m_all, -1, dtype=np.int32)
-    col_map[cols] = np.arange(len(cols), dtype=np.int32)
-    cm = torch.from_numpy(col_map).to(X.device)
+    cols = np.asarray(cols, dtype=np.int64)
+    if cols.size and (cols[0] < 0 or cols[-1] >= m_all or np.any(np.diff(cols) <= 0)):
+        raise ValueError("cols must be distinct ascending column ids of X")
+    d_cols = torch.from_numpy(cols).to(X.device)
+    sel = torch.empty(2 * (-(-m_all // 32)), dtype=torch.int32, device=X.device)
+    st = _lib.stream_ptr()
+    _lib.check(L.snapgpu_col_select_table(d_cols.data_ptr(), cols.size, m_all, sel.data_ptr(), st))
     bits = 32 if X.indices.dtype == torch.int32 else 64
     indptr = torch.zeros(n + 1, dtype=torch.int64, device=X.device)
-    st = _lib.stream_ptr()
     _lib.check(L.snapgpu_csr_select_count(X.indptr.data_ptr(), X.indices.data_ptr(), bits, n,
-                                          cm.data_ptr(), indptr.data_ptr() + 8, st))
+                                          sel.data_ptr(), m_all, indptr.data_ptr() + 8, st))
     ws_b = int(L.snapgpu_scan_workspace_bytes(n))
     ws = torch.empty(max(ws_b, 8), dtype=torch.uint8, device=X.device)
     _lib.check(L.snapgpu_scan_inclusive_i64(indptr.data_ptr() + 8, n, ws.data_ptr(), ws_b, st))
@@ -61,7 +64,7 @@ def select_columns(X: DeviceCSR, cols: np.ndarray | None) -> DeviceCSR:
     idx = torch.empty(nnz, dtype=torch.int32, device=X.device)
     val = torch.empty(nnz, dtype=torch.int32, device=X.device)
     _lib.check(L.snapgpu_csr_select_fill(X.indptr.data_ptr(), X.indices.data_ptr(), bits,
-                                         X.data.data_ptr(), n, cm.data_ptr(), indptr.data_ptr(),
+                                         X.data.data_ptr(), n, sel.data_ptr(), m_all, indptr.data_ptr(),
                                          idx.data_ptr(), val.data_ptr(), st))
     return DeviceCSR(indptr, idx, val, (n, len(cols)), X.data_dtype)
 

@@ -119,3 +119,30 @@ def test_transpose_counts(torch_gpu, shuffle):
     assert np.array_equal(op.colptr.cpu().numpy(), Xc.indptr)
     assert np.array_equal(op.rowidx.cpu().numpy(), Xc.indices)
     assert np.array_equal(op.cval.cpu().numpy().view(np.uint32), Xc.data)
+
+
+@pytest.mark.parametrize("m,frac,bits", [(1, 1.0, 32), (31, 0.5, 32), (33, 0.3, 64), (70001, 0.1, 32),
+                                         (100000, 0.0, 32), (65, 1.0, 64)])
+def test_select_columns_matches_scipy(torch_gpu, m, frac, bits):
+    """select_columns (selection table of 2 words per 32 columns) against
+    scipy column slicing: same rows, renumbered columns, same values."""
+    torch = torch_gpu
+    from snapatac2_amd._device import DeviceCSR
+    from snapatac2_amd.tl._operator import select_columns
+    n = 3000
+    indptr, idx, val = _random_csr(n, m, min(m, 40), seed=m, vmax=100)
+    rng = np.random.default_rng(m)
+    cols = np.flatnonzero(rng.random(m) < frac)
+    if frac == 1.0:
+        cols = np.arange(m)
+    dev = torch.device("cuda")
+    D = DeviceCSR(torch.from_numpy(indptr).to(dev),
+                  torch.from_numpy(idx.astype(np.int32 if bits == 32 else np.int64)).to(dev),
+                  torch.from_numpy(val.view(np.int32)).to(dev), (n, m))
+    S = select_columns(D, cols)
+    ref = sp.csr_matrix((val, idx, indptr), shape=(n, m))[:, cols]
+    ref.sort_indices()
+    assert S.shape == ref.shape
+    assert np.array_equal(S.indptr.cpu().numpy(), ref.indptr)
+    assert np.array_equal(S.indices.cpu().numpy(), ref.indices)
+    assert np.array_equal(S.data.cpu().numpy().view(np.uint32), ref.data)
