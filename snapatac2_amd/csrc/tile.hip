@@ -322,11 +322,24 @@ __device__ void process_cell(const Params& p, const ChromRec* tab, WaveLds<CAP>*
   const bool paired = p.strategy != SNAPGPU_SINGLE;
   int hint = 0;
 
+  // the next tile's positions and lengths are loaded one tile ahead (clamped
+  // to the cell's last fragment, so every load is in bounds and unconditional)
+  uint32_t ps_n = 0u, ln_n = 1u;
+  if (f0 < f1) {
+    const int64_t fc = min(f0 + lane, f1 - 1);
+    ps_n = p.pos[fc];
+    if (paired) ln_n = p.len[fc];
+  }
   for (int64_t t0 = f0; t0 < f1; t0 += 64) {
     const int64_t f = t0 + lane;
     const bool has = f < f1;
-    uint32_t ps = has ? p.pos[f] : 0u;
-    uint32_t ln = (has && paired) ? p.len[f] : 1u;
+    const uint32_t ps = has ? ps_n : 0u;
+    const uint32_t ln = (has && paired) ? ln_n : 1u;
+    {
+      const int64_t fc = min(f + 64, f1 - 1);
+      ps_n = p.pos[fc];
+      if (paired) ln_n = p.len[fc];
+    }
     bool kept = false;
     uint32_t b1 = kInf, b2 = kInf;
     if (has) {
@@ -359,7 +372,7 @@ __device__ void process_cell(const Params& p, const ChromRec* tab, WaveLds<CAP>*
     // Lower bound T of every event of the fragments after this tile.
     uint32_t T = kInf;
     if (t0 + 64 < f1) {
-      const uint32_t pn = p.pos[t0 + 64];
+      const uint32_t pn = (uint32_t)__shfl((int)ps_n, 0, 64);  // position of fragment t0 + 64
       T = lower_bound_bin(tab, nc, pn, hint, p);
     }
 
