@@ -259,8 +259,8 @@ def main():
                     help="first run one copy kernel of MIB MiB (calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box here has a single MI355X; "
-              "multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
+        print("bench.py measures one GPU only (one MI355X); sharding the path over several "
+              "GPUs is a design, not something this script runs", file=sys.stderr)
         sys.exit(2)
     import torch
     from snapatac2_amd import synth
