@@ -207,7 +207,12 @@ __global__ void __launch_bounds__(256) product_kernel(const uint32_t* __restrict
   const int lane = threadIdx.x & 63;
   const int g = lane / BW;
   const int c = lane % BW;
-  const int64_t chunk = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * G + g;
+  // Blocks go to the 8 XCDs round-robin; give each XCD a contiguous run of
+  // group-chunks, so the chunks resident on one XCD share a panel and its L2.
+  const int64_t nb = gridDim.x, q8 = nb / 8, r8 = nb % 8;
+  const int64_t xcd = blockIdx.x % 8;
+  const int64_t blk = xcd * q8 + min(xcd, r8) + blockIdx.x / 8;
+  const int64_t chunk = ((blk * blockDim.x + threadIdx.x) >> 6) * G + g;
   int64_t pos = 0, end = 0;
   int64_t k = 0;
   const float* db = dense;
