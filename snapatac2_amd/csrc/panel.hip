@@ -274,8 +274,20 @@ __global__ void __launch_bounds__(256) reduce_kernel(const int64_t* __restrict__
   const int64_t row = t / BW;
   const int c = (int)(t % BW);
   if (row >= nrows) return;
+  // pairs in panel order; four loads in flight, added in the same order
+  const int64_t i0 = rp_ptr[row], i1 = rp_ptr[row + 1];
   float acc = 0.f;
-  for (int64_t i = rp_ptr[row]; i < rp_ptr[row + 1]; ++i) acc += partial[(int64_t)rp_ids[i] * BW + c];
+  int64_t i = i0;
+  for (; i + 4 <= i1; i += 4) {
+    const int32_t p0 = rp_ids[i], p1 = rp_ids[i + 1], p2 = rp_ids[i + 2], p3 = rp_ids[i + 3];
+    const float a0 = partial[(int64_t)p0 * BW + c], a1 = partial[(int64_t)p1 * BW + c];
+    const float a2 = partial[(int64_t)p2 * BW + c], a3 = partial[(int64_t)p3 * BW + c];
+    acc += a0;
+    acc += a1;
+    acc += a2;
+    acc += a3;
+  }
+  for (; i < i1; ++i) acc += partial[(int64_t)rp_ids[i] * BW + c];
   if (EPI == 0) {
     out32[row * BW + c] = acc * scale[row];
   } else if (c < b) {

@@ -735,11 +735,24 @@ __global__ void adj_rows_kernel(const double* G, int64_t ldg, const double* V, i
   t[i] = ga * u[i];
 }
 
-// y1_j = sum_i cnt_ij^POW x1_i (and y2 from x2 when given); one wave per column.
+// pair[i] = (a[i], b[i]): two vectors gathered with one 16-byte load
+__global__ void pack_pair_kernel(const double* a, const double* b, int64_t n, double2* pair) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) pair[i] = make_double2(a[i], b[i]);
+}
+
+// pair[j] = (w_j^2, x_j)
+__global__ void pack_w2x_kernel(const double* w, const double* x, int64_t m, double2* pair) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) pair[j] = make_double2(w[j] * w[j], x[j]);
+}
+
+// y1_j = sum_i cnt_ij^POW x1_i (and y2_j from the second component when x12 is
+// given instead: x12[i] = (x1_i, x2_i)); one wave per column.
 template <bool FV, int POW>
 __global__ void __launch_bounds__(256) csc_spmv2_kernel(const int64_t* colptr, const int32_t* rowidx,
                                                         const uint32_t* cnt, int64_t m, const double* x1,
-                                                        const double* x2, double* y1, double* y2) {
+                                                        const double2* x12, double* y1, double* y2) {
   const int lane = threadIdx.x & 63;
   const int64_t col = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (col >= m) return;
@@ -748,8 +761,13 @@ __global__ void __launch_bounds__(256) csc_spmv2_kernel(const int64_t* colptr, c
     double v = val64<FV>(cnt, e);
     if (POW == 2) v *= v;
     const int32_t i = rowidx[e];
-    a1 = fma(v, x1[i], a1);
-    if (x2 != nullptr) a2 = fma(v, x2[i], a2);
+    if (x12 != nullptr) {
+      const double2 x = x12[i];
+      a1 = fma(v, x.x, a1);
+      a2 = fma(v, x.y, a2);
+    } else {
+      a1 = fma(v, x1[i], a1);
+    }
   }
   for (int o = 32; o > 0; o >>= 1) {
     a1 += __shfl_xor(a1, o, 64);
@@ -761,18 +779,18 @@ __global__ void __launch_bounds__(256) csc_spmv2_kernel(const int64_t* colptr, c
   }
 }
 
-// y_i = sum_j cnt_ij w_j^2 x_j; one wave per row.
+// y_i = sum_j cnt_ij w_j^2 x_j with w2x[j] = (w_j^2, x_j); one wave per row.
 template <bool FV>
 __global__ void __launch_bounds__(256) csr_spmv_w2_kernel(const int64_t* indptr, const int32_t* idx,
-                                                          const uint32_t* cnt, int64_t n, const double* w,
-                                                          const double* x, double* y) {
+                                                          const uint32_t* cnt, int64_t n, const double2* w2x,
+                                                          double* y) {
   const int lane = threadIdx.x & 63;
   const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (row >= n) return;
   double a = 0.0;
   for (int64_t e = indptr[row] + lane; e < indptr[row + 1]; e += 64) {
-    const int32_t j = idx[e];
-    a = fma(val64<FV>(cnt, e) * (w[j] * w[j]), x[j], a);
+    const double2 q = w2x[idx[e]];
+    a = fma(val64<FV>(cnt, e) * q.x, q.y, a);
   }
   for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
   if (lane == 0) y[row] = a;
@@ -864,8 +882,13 @@ static void launch_vjp(const int64_t* indptr, const int32_t* idx, const uint32_t
                                                                 alpha, gamma, t);
   }
   // phi = X^T (gamma o u) and e = X^T u in one pass over the CSC
-  csc_spmv2_kernel<FV, 1><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, t, u, phi, e);
-  csr_spmv_w2_kernel<FV><<<gn, 256, 0, st>>>(indptr, idx, cnt, n, w, phi, psi);
+  // (t, u) side by side in Gs and (w^2, phi) in Qm, both free by now
+  double2* tu = reinterpret_cast<double2*>(Gs);
+  double2* w2phi = reinterpret_cast<double2*>(Qm);
+  pack_pair_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(t, u, n, tu);
+  csc_spmv2_kernel<FV, 1><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, nullptr, tu, phi, e);
+  pack_w2x_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(w, phi, m, w2phi);
+  csr_spmv_w2_kernel<FV><<<gn, 256, 0, st>>>(indptr, idx, cnt, n, w2phi, psi);
   kappa_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(n, u, dinv, alpha, gamma, psi, kappa);
   csc_spmv2_kernel<FV, 2><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, kappa, nullptr, sigma, nullptr);
   dw_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(m, w, h, e, phi, sigma, dw);
