@@ -166,6 +166,42 @@ def spmm_roofline(torch, op, events, block, workload):
             "avg_ms": round(avg_ms, 3), "launches": len(durs), "alg_bytes": int(alg)}
 
 
+def gather_ceiling(torch, roof, nnz):
+    """The operator's gather rate against the probe's: both sparse products
+    gather one 16-float row per stored entry (2 nnz per application), and
+    snapgpu_gather_probe times the same gathers, with the same index stream and
+    lane layout, alone from one L2-sized panel (32,768 rows = 2 MB) at 2^28
+    uniformly random rows."""
+    from snapatac2_amd import _lib
+    if roof is None:
+        return None
+    L = _lib.lib()
+    rows, count = 32768, 1 << 28
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    idx = torch.randint(0, rows, (count,), dtype=torch.int32, device="cuda", generator=gen)
+    dense = torch.randn(rows * 16, dtype=torch.float32, device="cuda", generator=gen)
+    out = torch.empty(4096 * 256, dtype=torch.float32, device="cuda")
+    st = _lib.stream_ptr()
+
+    def run():
+        _lib.check(L.snapgpu_gather_probe(idx.data_ptr(), count, dense.data_ptr(), rows, out.data_ptr(),
+                                          out.numel(), st))
+    for _ in range(2):
+        run()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    probe = count / (e0.elapsed_time(e1) / 5 * 1e-3)
+    op_rate = 2 * nnz / (roof["avg_ms"] * 1e-3)
+    return {"unit": "G gathers/s", "probe": round(probe / 1e9, 1), "operator": round(op_rate / 1e9, 1),
+            "frac": round(op_rate / probe, 3),
+            "note": "one 64-byte row gather per stored entry and product; probe = the same gathers alone"}
+
+
 def cpu_baseline(torch, chunks, cfg, emb, sel_nnz_full):
     """Reference algorithm (oracle restatements) timed on this host's cores on a
     bounded sample, extrapolated to the workload's cell count."""
@@ -333,6 +369,9 @@ def main():
     hits = sum(np.intersect1d(g.indices[g.indptr[i]:g.indptr[i + 1]], ex).size
                for i, ex in zip(qi, exact))
     recall = hits / (len(qi) * cfg[3])
+    gceil = gather_ceiling(torch, roof, info["selected_nnz"])
+    if gceil:
+        log(f"[bench] gathers: operator {gceil['operator']} G/s, probe {gceil['probe']} G/s")
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline ...")
@@ -355,6 +394,7 @@ def main():
                    "tile_nnz": info["tile_nnz"], "selected_nnz": info["selected_nnz"],
                    "parallelism": "single GPU"},
         "roofline": roof,
+        "gather_ceiling": gceil,
         "cpu_baseline": cpu,
         "spmm_gbps": roof["achieved"] if roof else None,
         "knn_recall_sample": recall,

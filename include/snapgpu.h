@@ -47,6 +47,15 @@ int snapgpu_abi_version(void);
  * counters that profiles report. */
 int snapgpu_copy_bytes(const void* d_src, void* d_dst, int64_t bytes, void* stream);
 
+/* Gather-rate probe (no reference counterpart; a measurement aid): reads one
+ * 16-float row of d_dense[rows][16] per entry of d_idx (row ids < rows), with
+ * the access pattern of the operator's sparse products (16 lanes per row, 64
+ * indices per group batch, eight rows in flight per lane). Timed by the caller;
+ * with an L2-sized d_dense it gives the gather ceiling of those products.
+ * d_out: 4096 * 256 floats (per-thread sums, so the loads stay live). */
+int snapgpu_gather_probe(const uint32_t* d_idx, int64_t count, const float* d_dense, int64_t rows,
+                         float* d_out, int64_t out_len, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Prefix sums                                                              */
 /* ------------------------------------------------------------------------ */

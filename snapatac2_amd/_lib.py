@@ -20,6 +20,7 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_last_error": (C.c_char_p, []),
     "snapgpu_abi_version": (C.c_int, []),
     "snapgpu_copy_bytes": (C.c_int, [_p, _p, _i64, _p]),
+    "snapgpu_gather_probe": (C.c_int, [_p, _i64, _p, _i64, _p, _i64, _p]),
     "snapgpu_scan_workspace_bytes": (_i64, [_i64]),
     "snapgpu_scan_inclusive_i64": (C.c_int, [_p, _i64, _p, _i64, _p]),
     "snapgpu_tile_table_bytes": (_i64, [_i32]),
