@@ -38,6 +38,10 @@ using spectral::scale_in_kernel;
 #endif
 constexpr int64_t kPanel = SNAPGPU_PANEL_ROWS;  // dense rows per panel (<= 32768: 15-bit column)
 constexpr int64_t kChunk = SNAPGPU_PANEL_CHUNK;  // entries per group-chunk (aligned to pair starts)
+#ifndef SNAPGPU_PANEL_INFLIGHT
+#define SNAPGPU_PANEL_INFLIGHT 8
+#endif
+constexpr int kInflight = SNAPGPU_PANEL_INFLIGHT;  // gathers in flight per lane
 static_assert(kPanel <= 32768, "the packed entry holds a 15-bit column");
 
 // seg[row][p] = offset within the row of its first entry with idx >= p * kPanel
@@ -183,7 +187,7 @@ __global__ void chunk_kernel(const int64_t* __restrict__ pstart, const int64_t* 
 // The product. Group g of the wave (b lanes, lane c = dense column c) streams
 // group-chunk wave * G + g: 4 * BW entries per batch, one coalesced dword per
 // lane per quarter; entry j of the batch is broadcast to the group from lane
-// j % BW, register j / BW. Gathers are issued 8 at a time. A pair's first entry
+// j % BW, register j / BW. Gathers are issued kInflight at a time. A pair's first entry
 // carries a flag: the running sum of the previous pair is stored and reset.
 // ---------------------------------------------------------------------------
 template <bool PACK, bool FV>
@@ -234,18 +238,18 @@ __global__ void __launch_bounds__(256) product_kernel(const uint32_t* __restrict
     }
     const int nvalid = (int)min((int64_t)NB, max((int64_t)0, end - pos));
 #pragma unroll
-    for (int j0 = 0; j0 < NB; j0 += 8) {
-      uint32_t ej[8], vj[8];
-      float z[8];
+    for (int j0 = 0; j0 < NB; j0 += kInflight) {
+      uint32_t ej[kInflight], vj[kInflight];
+      float z[kInflight];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < kInflight; ++u) {
         const int j = j0 + u;
         ej[u] = (uint32_t)__shfl((int)e[j / BW], g * BW + j % BW, 64);
         vj[u] = PACK ? 0u : (uint32_t)__shfl((int)v[j / BW], g * BW + j % BW, 64);
         z[u] = j < nvalid ? db[(int64_t)(ej[u] & kCol) * BW + c] : 0.f;
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < kInflight; ++u) {
         if (ej[u] & kFirst) {
           if (have) partial[k++ * BW + c] = acc;
           have = true;
