@@ -202,6 +202,31 @@ def gather_ceiling(torch, roof, nnz):
             "note": "one 64-byte row gather per stored entry and product; probe = the same gathers alone"}
 
 
+def knn_precision(torch, n, k=50, d=30):
+    """Exact kNN on 10 Gaussian clusters in d dims (centres 3 N(0, I), points
+    centre + N(0, I), rows L2-normalised): BF16 MFMA candidates against FP32
+    SIMT candidates, both rescored in FP64. Returns the second of two timings of
+    each and whether the two graphs are identical."""
+    from snapatac2_amd.pp._knn import nearest_neighbour_graph
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    centres = 3.0 * torch.randn(10, d, dtype=torch.float64, device="cuda", generator=gen)
+    lab = torch.randint(0, 10, (n,), device="cuda", generator=gen)
+    X = centres[lab] + torch.randn(n, d, dtype=torch.float64, device="cuda", generator=gen)
+    X = (X / torch.linalg.norm(X, dim=1, keepdim=True)).to(torch.float32).to(torch.float64)
+    t, g = {}, {}
+    for prec in ("bf16", "fp32", "bf16", "fp32"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g[prec] = nearest_neighbour_graph(X, k, precision=prec)
+        torch.cuda.synchronize()
+        t[prec] = time.perf_counter() - t0
+    same = bool(np.array_equal(g["bf16"].indices, g["fp32"].indices)
+                and np.array_equal(g["bf16"].data, g["fp32"].data))
+    return {"workload": f"{n} x {d} clustered, row-normalised, k = {k}", "bf16_s": round(t["bf16"], 3),
+            "fp32_s": round(t["fp32"], 3), "bf16_speedup": round(t["fp32"] / t["bf16"], 2),
+            "identical_graphs": same}
+
+
 def cpu_baseline(torch, chunks, cfg, emb, sel_nnz_full):
     """Reference algorithm (oracle restatements) timed on this host's cores on a
     bounded sample, extrapolated to the workload's cell count."""
@@ -370,6 +395,8 @@ def main():
                for i, ex in zip(qi, exact))
     recall = hits / (len(qi) * cfg[3])
     gceil = gather_ceiling(torch, roof, info["selected_nnz"])
+    kprec = knn_precision(torch, n_cells)
+    log(f"[bench] kNN precision: {kprec}")
     if gceil:
         log(f"[bench] gathers: operator {gceil['operator']} G/s, probe {gceil['probe']} G/s")
     cpu = None
@@ -395,6 +422,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roof,
         "gather_ceiling": gceil,
+        "knn_precision": kprec,
         "cpu_baseline": cpu,
         "spmm_gbps": roof["achieved"] if roof else None,
         "knn_recall_sample": recall,
