@@ -325,7 +325,7 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
 // the centre, given their augmented coordinate |x - c|^2, split into three
 // bf16 terms and staged in LDS; every wave then runs its 16 x 16 MFMA tiles.
 __global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredParams P) {
-  __shared__ __bf16 As[3][kChunk][kDim];
+  __shared__ __align__(16) __bf16 As[3][kChunk][kDim];
   __shared__ float cen[kDim];
   __shared__ float red[kWgWaves];
   __shared__ unsigned long long vmask;
@@ -404,36 +404,55 @@ __global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredPara
     }
     __syncthreads();
     unsigned long long m = vmask;
+    // FP64 coordinates of this thread's point and 8 dimensions of the next chunk
+    // to stage, loaded while the current chunk's tiles run (xs is zero-padded
+    // to whole chunks, so the loads need no bounds test)
+    double xr[8];
+#define SG_PREFETCH(mask)                                                              \
+  {                                                                                    \
+    const int64_t cp = visit_chunk(v0 + __ffsll(mask) - 1, wg, P.n_wg) * kChunk;      \
+    const double2* src = reinterpret_cast<const double2*>(P.xs + (cp + sp) * kDim + sq * 8); \
+    _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                    \
+      const double2 t2 = src[e];                                                       \
+      xr[2 * e] = t2.x;                                                                \
+      xr[2 * e + 1] = t2.y;                                                            \
+    }                                                                                  \
+  }
+    if (m != 0) SG_PREFETCH(m)
     while (m != 0) {
       const int bit = __ffsll(m) - 1;
       m &= m - 1;
       const int64_t c0 = visit_chunk(v0 + bit, wg, P.n_wg) * kChunk;
       ++visited;
       __syncthreads();
-      for (int pp = sp; pp < kChunk; pp += (kWgWaves * 64) / 4) {
-        const int64_t p = c0 + pp;
+      {
+        const int64_t p = c0 + sp;
         float v[8];
         float ps = 0.f;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           const int c = sq * 8 + e;
-          v[e] = 0.f;
-          if (p < P.n && c < P.d) v[e] = (float)(P.xs[p * kDim + c] - (double)cen[c]);
+          v[e] = (p < P.n && c < P.d) ? (float)(xr[e] - (double)cen[c]) : 0.f;
           ps = fmaf(v[e], v[e], ps);
         }
         ps += __shfl_xor(ps, 1, 64);
         ps += __shfl_xor(ps, 2, 64);
         if (sq == 3) v[6] = ps;  // dim 30 = |x - c|^2
+        bf16x8 th, tm, tl;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           __bf16 h, mm, l;
           split3(v[e], h, mm, l);
-          As[0][pp][sq * 8 + e] = h;
-          As[1][pp][sq * 8 + e] = mm;
-          As[2][pp][sq * 8 + e] = l;
+          th[e] = h;
+          tm[e] = mm;
+          tl[e] = l;
         }
+        *reinterpret_cast<bf16x8*>(&As[0][sp][sq * 8]) = th;
+        *reinterpret_cast<bf16x8*>(&As[1][sp][sq * 8]) = tm;
+        *reinterpret_cast<bf16x8*>(&As[2][sp][sq * 8]) = tl;
       }
       __syncthreads();
+      if (m != 0) SG_PREFETCH(m)
 #pragma unroll 2
       for (int tt = 0; tt < kChunk / 16; ++tt) {
         const int row = tt * 16 + j;
@@ -461,6 +480,7 @@ __global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredPara
       }
     }
   }
+#undef SG_PREFETCH
   if (tid == 0 && P.visited) atomicAdd(P.visited, visited);
   if (!qvalid) return;
   int32_t* out = P.cand + qi * kCand + g * kList;
@@ -777,8 +797,14 @@ struct SegParams {
   float* tau;     // [n_rows][S][4]  (d^2 bound per stripe)
 };
 
+// Each wave streams its segment in tiles of 16 points: the 16 x 32 FP64 tile is
+// loaded coalesced (8 values per lane, the next tile's loaded while the current
+// one is scored) and shared through the wave's LDS slice; lane (query j,
+// stripe g) scores points 4g .. 4g + 3 of the tile.
 __global__ void __launch_bounds__(256) exact_seg_kernel(SegParams P) {
+  __shared__ double tile[4][16][kDim + 1];
   const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int seg = blockIdx.y;
   const int j = lane & 15, g = lane >> 4;
@@ -798,21 +824,42 @@ __global__ void __launch_bounds__(256) exact_seg_kernel(SegParams P) {
   }
   const int64_t p0 = (int64_t)seg * P.seg_size;
   const int64_t p1 = min(P.n, p0 + P.seg_size);
+  // element k = lane + 64 i of a tile: point k / 32, dimension k % 32; clamped
+  // reads (values of points past p1 or dimensions past d are never used)
+  double xn[8];
+  auto load_tile = [&](int64_t t0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = lane + 64 * i;
+      const int64_t p = min(t0 + (k >> 5), P.n - 1);
+      const int c = min(k & 31, P.d - 1);
+      xn[i] = P.X[p * P.ld + c];
+    }
+  };
+  if (p0 < p1) load_tile(p0);
   for (int64_t t0 = p0; t0 < p1; t0 += 16) {
+    wave_sync();  // previous tile's reads are done
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = lane + 64 * i;
+      tile[wv][k >> 5][k & 31] = xn[i];
+    }
+    wave_sync();
+    if (t0 + 16 < p1) load_tile(t0 + 16);
     const float th = L[kList - 1];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int64_t p = t0 + g * 4 + v;
-      float sv = -INFINITY;
-      if (p < p1) {
-        const double* xr = P.X + p * P.ld;
-        double acc = 0.0;
-        for (int c = 0; c < P.d; ++c) {
-          const double df = xr[c] - qv[c];
+      const int pt = g * 4 + v;
+      const int64_t p = t0 + pt;
+      double acc = 0.0;
+#pragma unroll
+      for (int c = 0; c < kMaxData; ++c) {
+        if (c < P.d) {
+          const double df = tile[wv][pt][c] - qv[c];
           acc = fma(df, df, acc);
         }
-        sv = -(float)acc;
       }
+      const float sv = p < p1 ? -(float)acc : -INFINITY;
       const bool ins = sv > th;
       if (__ballot(ins)) {
         if (ins) list_insert(L, I, sv, (int32_t)p);

@@ -95,6 +95,19 @@ def test_second_tier_on_offset_clusters(torch_gpu):
     assert st["fallback_rows"] <= 30
 
 
+@pytest.mark.parametrize("n,d", [(2999, 7), (1001, 30), (17, 1)])
+def test_second_tier_shapes(torch_gpu, n, d):
+    """Offset clusters push rows to the exact FP64 tier; point counts that are
+    not a multiple of the 16-point tile and odd dimensions exercise its tile
+    loads and padding."""
+    from snapatac2_amd.pp._knn import nearest_neighbour_graph
+    X = 1.0 + 1e-3 * clustered(n, d, seed=n + d)
+    k = min(50, n - 1)
+    g, st = nearest_neighbour_graph(X, k, precision="fp32", return_stats=True)
+    _check(g, X, k)
+    assert st["exact_scan_rows"] <= st["fallback_rows"]
+
+
 @pytest.mark.parametrize("n_clusters,spread", [(10, 3.0), (40, 8.0), (1, 0.0)])
 def test_chunk_skipping_is_exact(torch_gpu, n_clusters, spread):
     """The bf16 path skips database chunks whose bounding sphere lies beyond every
