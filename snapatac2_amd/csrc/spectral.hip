@@ -118,19 +118,17 @@ __global__ void col_count_kernel(const int32_t* idx, int64_t nnz, unsigned long 
     atomicAdd(&counts[idx[e]], 1ull);
 }
 
-// row id of every entry of rows [r0, r1) (entries [indptr[r0], indptr[r1]))
-__global__ void __launch_bounds__(256) entry_rows_kernel(const int64_t* indptr, int64_t r0, int64_t r1,
-                                                         uint32_t* row_of) {
+// (row << 32 | value) of every entry of rows [r0, r1) (entries [indptr[r0], indptr[r1])):
+// the sort by column carries row and value along, so placing an entry reads them
+// in sorted order instead of gathering them by original position
+__global__ void __launch_bounds__(256) entry_pack_kernel(const int64_t* indptr, int64_t r0, int64_t r1,
+                                                         const uint32_t* val, uint64_t* packed) {
   const int lane = threadIdx.x & 63;
   const int64_t row = r0 + (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (row >= r1) return;
   const int64_t base = indptr[r0];
-  for (int64_t e = indptr[row] + lane; e < indptr[row + 1]; e += 64) row_of[e - base] = (uint32_t)row;
-}
-
-__global__ void iota_kernel(uint32_t* v, int64_t n) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) v[i] = (uint32_t)i;
+  for (int64_t e = indptr[row] + lane; e < indptr[row + 1]; e += 64)
+    packed[e - base] = ((uint64_t)row << 32) | (uint64_t)val[e];
 }
 
 // Runs of equal keys in a sorted chunk: cursor[col] is the next free CSC slot of
@@ -146,15 +144,14 @@ __global__ void cursor_rebase_kernel(const uint32_t* keys, int64_t cn, int64_t* 
   }
 }
 
-__global__ void scatter_chunk_kernel(const uint32_t* skeys, const uint32_t* spos, int64_t cn,
-                                     const uint32_t* row_of, const uint32_t* cval,
+__global__ void scatter_chunk_kernel(const uint32_t* skeys, const uint64_t* spacked, int64_t cn,
                                      const int64_t* cursor, int32_t* rowidx, uint32_t* out_val) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cn; t += stride) {
     const int64_t o = cursor[skeys[t]] + t;
-    const uint32_t p = spos[t];
-    rowidx[o] = (int32_t)row_of[p];
-    out_val[o] = cval[p];
+    const uint64_t v = spacked[t];
+    rowidx[o] = (int32_t)(v >> 32);
+    out_val[o] = (uint32_t)v;
   }
 }
 
@@ -959,9 +956,9 @@ int snapgpu_csr_select_fill(const int64_t* d_indptr, const void* d_idx, int32_t 
 int64_t snapgpu_csr_transpose_workspace_bytes(int64_t m, int64_t chunk_nnz) {
   size_t tmp = 0;
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                           (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (const uint64_t*)nullptr, (uint64_t*)nullptr,
                                            (int)std::min<int64_t>(chunk_nnz, INT32_MAX));
-  return (int64_t)m * 8 + 256 + chunk_nnz * 4 * 4 + 5 * 256 + (int64_t)tmp + 256;
+  return (int64_t)m * 8 + 256 + chunk_nnz * 24 + 5 * 256 + (int64_t)tmp + 256;
 }
 
 // Stable transpose. colptr[m+1] out; rows ascending inside each column.
@@ -983,8 +980,8 @@ int snapgpu_csr_transpose(const int64_t* d_indptr, const int32_t* d_idx, const u
   auto take = [&](int64_t bytes) { char* q = p; p += (bytes + 255) / 256 * 256; return q; };
   uint32_t* keys_in = (uint32_t*)take(chunk_nnz * 4);
   uint32_t* keys_out = (uint32_t*)take(chunk_nnz * 4);
-  uint32_t* pos_in = (uint32_t*)take(chunk_nnz * 4);
-  uint32_t* pos_out = (uint32_t*)take(chunk_nnz * 4);
+  uint64_t* vals_in = (uint64_t*)take(chunk_nnz * 8);
+  uint64_t* vals_out = (uint64_t*)take(chunk_nnz * 8);
   int32_t* wstatus = (int32_t*)take(4);
   void* tmp = p;
   size_t tmp_bytes = (size_t)(ws_bytes - (p - (char*)d_ws));
@@ -1025,17 +1022,14 @@ int snapgpu_csr_transpose(const int64_t* d_indptr, const int32_t* d_idx, const u
     const int64_t e0 = h_indptr[r0], cn = h_indptr[r1] - e0;
     if (cn > 0) {
       SG_HIP(hipMemcpyAsync(keys_in, d_idx + e0, cn * 4, hipMemcpyDeviceToDevice, st));
-      iota_kernel<<<grid_for(cn, 256, 16384), 256, 0, st>>>(pos_in, cn);
+      entry_pack_kernel<<<(unsigned)ceil_div(r1 - r0, 4), 256, 0, st>>>(d_indptr, r0, r1, d_val, vals_in);
       SG_LAUNCH_CHECK();
-      SG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, keys_out, pos_in, pos_out,
+      SG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, keys_out, vals_in, vals_out,
                                                 (int)cn, 0, end_bit, st));
-      // keys_in is free again: reuse it for the row ids of the chunk's entries
-      entry_rows_kernel<<<(unsigned)ceil_div(r1 - r0, 4), 256, 0, st>>>(d_indptr, r0, r1, keys_in);
-      SG_LAUNCH_CHECK();
       cursor_rebase_kernel<<<grid_for(cn, 256, 16384), 256, 0, st>>>(keys_out, cn, cursor);
       SG_LAUNCH_CHECK();
-      scatter_chunk_kernel<<<grid_for(cn, 256, 16384), 256, 0, st>>>(keys_out, pos_out, cn, keys_in,
-                                                                     d_val + e0, cursor, d_rowidx, d_cval);
+      scatter_chunk_kernel<<<grid_for(cn, 256, 16384), 256, 0, st>>>(keys_out, vals_out, cn, cursor, d_rowidx,
+                                                                     d_cval);
       SG_LAUNCH_CHECK();
       cursor_advance_kernel<<<grid_for(cn, 256, 16384), 256, 0, st>>>(keys_out, cn, cursor);
       SG_LAUNCH_CHECK();
