@@ -92,6 +92,14 @@ int snapgpu_tile_table_pack(const uint64_t* h_chrom_len, const uint8_t* h_exclud
                             int32_t n_chrom, uint32_t bin_size, void* h_out,
                             uint64_t* n_bins);
 
+/* Processing order for the two passes: the cells by decreasing fragment count
+ * (a wave walks one cell, so long cells started late would trail the launch).
+ * Same output in any order. d_order[n_cells] out; workspace from
+ * snapgpu_tile_order_workspace_bytes. */
+int64_t snapgpu_tile_order_workspace_bytes(int64_t n_cells);
+int snapgpu_tile_order(const uint64_t* d_cell_ptr, int64_t n_cells, uint32_t* d_order, void* d_ws,
+                       int64_t ws_bytes, void* stream);
+
 /* Pass 1: number of stored (bin, count) entries of each cell.
  *   d_cell_ptr[n_cells+1]: fragment row pointers of obsm['fragment_paired'|
  *     'fragment_single'] (import.rs:117-159);
@@ -100,6 +108,7 @@ int snapgpu_tile_table_pack(const uint64_t* h_chrom_len, const uint8_t* h_exclud
  *     used by gen_mat_single);
  *   min_len/max_len: inclusive size filter (0 / UINT32_MAX = none);
  *   d_row_nnz[n_cells]: output;
+ *   d_order: processing order from snapgpu_tile_order, or NULL (cell order);
  *   d_status[8]: int32 device scratch for error reporting;
  *   *level: in = first capacity level to try (0), out = level that succeeded
  *     (pass it to snapgpu_tile_fill).
@@ -108,7 +117,7 @@ int snapgpu_tile_count(const uint64_t* d_cell_ptr, const uint32_t* d_pos,
                        const uint32_t* d_len, int64_t n_cells, const void* d_table,
                        int32_t n_chrom, uint32_t bin_size, int32_t strategy,
                        uint32_t min_len, uint32_t max_len, int64_t* d_row_nnz,
-                       int32_t* d_status, int32_t* level, void* stream);
+                       const uint32_t* d_order, int32_t* d_status, int32_t* level, void* stream);
 
 /* Pass 2: writes the CSR columns (ascending per row) and counts.
  *   d_indptr[n_cells+1]: row pointers (scan of pass-1 counts);
@@ -119,7 +128,8 @@ int snapgpu_tile_fill(const uint64_t* d_cell_ptr, const uint32_t* d_pos,
                       int32_t n_chrom, uint32_t bin_size, int32_t strategy,
                       uint32_t min_len, uint32_t max_len, int32_t level,
                       const int64_t* d_indptr, void* d_out_idx, int32_t index_bits,
-                      uint32_t* d_out_val, int32_t* d_status, void* stream);
+                      uint32_t* d_out_val, const uint32_t* d_order, int32_t* d_status,
+                      void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Feature selection input                                                  */

@@ -25,10 +25,12 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_scan_inclusive_i64": (C.c_int, [_p, _i64, _p, _i64, _p]),
     "snapgpu_tile_table_bytes": (_i64, [_i32]),
     "snapgpu_tile_table_pack": (C.c_int, [_p, _p, _i32, _u32, _p, C.POINTER(_u64)]),
-    "snapgpu_tile_count": (C.c_int, [_p, _p, _p, _i64, _p, _i32, _u32, _i32, _u32, _u32, _p, _p,
+    "snapgpu_tile_order_workspace_bytes": (_i64, [_i64]),
+    "snapgpu_tile_order": (C.c_int, [_p, _i64, _p, _p, _i64, _p]),
+    "snapgpu_tile_count": (C.c_int, [_p, _p, _p, _i64, _p, _i32, _u32, _i32, _u32, _u32, _p, _p, _p,
                                      C.POINTER(_i32), _p]),
     "snapgpu_tile_fill": (C.c_int, [_p, _p, _p, _i64, _p, _i32, _u32, _i32, _u32, _u32, _i32, _p,
-                                    _p, _i32, _p, _p, _p]),
+                                    _p, _i32, _p, _p, _p, _p]),
     "snapgpu_csr_col_sum_u32": (C.c_int, [_p, _i32, _p, _i64, _p, _p]),
     "snapgpu_csr_col_sum_rows_u32": (C.c_int, [_p, _i64, _p, _i32, _p, _i64, _p, _p, _p]),
     "snapgpu_synth_counts": (C.c_int, [_u64, _i64, _i64, C.c_double, C.c_double, _i64, _i64, _p,

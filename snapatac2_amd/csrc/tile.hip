@@ -22,6 +22,8 @@
 // interleave, the usual case, else by a register bitonic network over the
 // wave), run-length encoded with ballots, and merged with the pending list by
 // rank (binary search in LDS). The same code runs twice: pass 1 only counts, pass 2 writes.
+#include <hipcub/hipcub.hpp>
+
 #include "common.h"
 
 namespace snapgpu {
@@ -42,6 +44,7 @@ struct Params {
   const uint64_t* cell_ptr;
   const uint32_t* pos;
   const uint32_t* len;  // null for single-end input
+  const uint32_t* order;  // processing order of the cells (null = 0, 1, ...)
   int64_t n_cells;
   const ChromRec* table;
   int32_t n_chrom;
@@ -467,8 +470,9 @@ __global__ void __launch_bounds__(WAVES * 64) tile_kernel(Params p) {
   for (int i = threadIdx.x; i <= p.n_chrom; i += blockDim.x) tab[i] = p.table[i];
   __syncthreads();
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int64_t cell = (int64_t)blockIdx.x * WAVES + wid;
-  if (cell >= p.n_cells) return;
+  const int64_t slot = (int64_t)blockIdx.x * WAVES + wid;
+  if (slot >= p.n_cells) return;
+  const int64_t cell = p.order ? (int64_t)p.order[slot] : slot;
   process_cell<CAP, FILL, IdxT>(p, tab, &waves[wid], cell, lane);
 }
 
@@ -536,6 +540,17 @@ static int check_status(const int32_t* d_status, hipStream_t st, bool* overflow)
   return SNAPGPU_OK;
 }
 
+// Sort keys for the largest-first order: one wave walks one cell, so a dispatch
+// lasts at least as long as its longest cell; started first, the long cells run
+// beside the bulk instead of forming a tail.
+__global__ void order_keys_kernel(const uint64_t* cell_ptr, int64_t n, uint32_t* keys, uint32_t* ids) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t len = cell_ptr[i + 1] - cell_ptr[i];
+  keys[i] = ~(uint32_t)min(len, (uint64_t)0xFFFFFFFFu);  // ascending keys = descending sizes
+  ids[i] = (uint32_t)i;
+}
+
 static int make_params(Params& p, const uint64_t* cell_ptr, const uint32_t* pos, const uint32_t* len,
                        int64_t n_cells, const void* table, int32_t n_chrom, uint32_t bin_size,
                        int32_t strategy, uint32_t min_len, uint32_t max_len, int32_t* status) {
@@ -600,15 +615,47 @@ int snapgpu_tile_table_pack(const uint64_t* h_chrom_len, const uint8_t* h_exclud
   return SNAPGPU_OK;
 }
 
+int64_t snapgpu_tile_order_workspace_bytes(int64_t n_cells) {
+  if (n_cells <= 0) return 256;
+  size_t tmp = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (int)std::min<int64_t>(n_cells, INT32_MAX));
+  return 3 * ((n_cells * 4 + 255) / 256 * 256) + (int64_t)tmp + 256;
+}
+
+int snapgpu_tile_order(const uint64_t* d_cell_ptr, int64_t n_cells, uint32_t* d_order, void* d_ws,
+                       int64_t ws_bytes, void* stream) {
+  clear_error();
+  SG_ARG(n_cells >= 0 && n_cells < (1ll << 31), "bad cell count");
+  if (n_cells == 0) return SNAPGPU_OK;
+  SG_ARG(d_cell_ptr && d_order && d_ws, "null pointer");
+  SG_ARG(ws_bytes >= snapgpu_tile_order_workspace_bytes(n_cells), "workspace too small");
+  hipStream_t st = as_stream(stream);
+  char* q = (char*)(((uintptr_t)d_ws + 255) & ~(uintptr_t)255);
+  const int64_t arr = (n_cells * 4 + 255) / 256 * 256;
+  uint32_t* keys = (uint32_t*)q;
+  uint32_t* keys2 = (uint32_t*)(q + arr);
+  uint32_t* ids = (uint32_t*)(q + 2 * arr);
+  void* tmp = q + 3 * arr;
+  size_t tmp_bytes = (size_t)(ws_bytes - (3 * arr + (q - (char*)d_ws)));
+  order_keys_kernel<<<(unsigned)ceil_div(n_cells, 256), 256, 0, st>>>(d_cell_ptr, n_cells, keys, ids);
+  SG_LAUNCH_CHECK();
+  SG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys2, ids, d_order, (int)n_cells, 0, 32,
+                                            st));
+  return SNAPGPU_OK;
+}
+
 int snapgpu_tile_count(const uint64_t* d_cell_ptr, const uint32_t* d_pos, const uint32_t* d_len,
                        int64_t n_cells, const void* d_table, int32_t n_chrom, uint32_t bin_size,
                        int32_t strategy, uint32_t min_len, uint32_t max_len, int64_t* d_row_nnz,
-                       int32_t* d_status, int32_t* level, void* stream) {
+                       const uint32_t* d_order, int32_t* d_status, int32_t* level, void* stream) {
   clear_error();
   Params p;
   int rc = make_params(p, d_cell_ptr, d_pos, d_len, n_cells, d_table, n_chrom, bin_size, strategy,
                        min_len, max_len, d_status);
   if (rc) return rc;
+  p.order = d_order;
   SG_ARG(level != nullptr && *level >= 0 && *level <= kMaxLevel, "bad level");
   SG_ARG(d_row_nnz != nullptr || n_cells == 0, "null row_nnz");
   if (n_cells == 0) return SNAPGPU_OK;
@@ -634,12 +681,13 @@ int snapgpu_tile_fill(const uint64_t* d_cell_ptr, const uint32_t* d_pos, const u
                       int64_t n_cells, const void* d_table, int32_t n_chrom, uint32_t bin_size,
                       int32_t strategy, uint32_t min_len, uint32_t max_len, int32_t level,
                       const int64_t* d_indptr, void* d_out_idx, int32_t index_bits,
-                      uint32_t* d_out_val, int32_t* d_status, void* stream) {
+                      uint32_t* d_out_val, const uint32_t* d_order, int32_t* d_status, void* stream) {
   clear_error();
   Params p;
   int rc = make_params(p, d_cell_ptr, d_pos, d_len, n_cells, d_table, n_chrom, bin_size, strategy,
                        min_len, max_len, d_status);
   if (rc) return rc;
+  p.order = d_order;
   SG_ARG(level >= 0 && level <= kMaxLevel, "bad level");
   SG_ARG(index_bits == 32 || index_bits == 64, "index_bits must be 32 or 64");
   if (n_cells == 0) return SNAPGPU_OK;

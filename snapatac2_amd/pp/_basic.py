@@ -89,9 +89,10 @@ def make_tile_matrix(
     cell_ptr = frags.cell_ptr
     level = C.c_int32(0)
     row_nnz_ptr = indptr.data_ptr() + 8
+    order = _cell_order(torch, L, cell_ptr, n, stream)
     _lib.check(L.snapgpu_tile_count(cell_ptr.data_ptr(), _lib.ptr(frags.pos), lens_ptr, n,
                                     table.data_ptr(), n_chrom, int(bin_size), strategy,
-                                    min_len, max_len, row_nnz_ptr, status.data_ptr(),
+                                    min_len, max_len, row_nnz_ptr, _lib.ptr(order), status.data_ptr(),
                                     C.byref(level), stream))
     ws_bytes = int(L.snapgpu_scan_workspace_bytes(n))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
@@ -104,12 +105,23 @@ def make_tile_matrix(
                                    table.data_ptr(), n_chrom, int(bin_size), strategy,
                                    min_len, max_len, level.value, indptr.data_ptr(),
                                    out_idx.data_ptr(), idx_bits, out_val.data_ptr(),
-                                   status.data_ptr(), stream))
+                                   _lib.ptr(order), status.data_ptr(), stream))
     kept = {c: chrom_sizes[c] for c in names if c not in excl}
     return DeviceCSR(indptr, out_idx, out_val, (n, n_bins)), BinIndex(kept, bin_size)
 
 
 _TILE_STREAMS = 4
+
+
+def _cell_order(torch, L, cell_ptr, n: int, stream):
+    """Cells by decreasing fragment count (snapgpu_tile_order), or None for no cells."""
+    if n == 0:
+        return None
+    ws_b = int(L.snapgpu_tile_order_workspace_bytes(n))
+    ws = torch.empty(ws_b, dtype=torch.uint8, device=cell_ptr.device)
+    order = torch.empty(n, dtype=torch.int32, device=cell_ptr.device)
+    _lib.check(L.snapgpu_tile_order(cell_ptr.data_ptr(), n, order.data_ptr(), ws.data_ptr(), ws_b, stream))
+    return order
 
 
 def make_tile_matrix_chunked(chunks, chrom_sizes: dict[str, int], bin_size: int = 500,
@@ -171,14 +183,17 @@ def make_tile_matrix_chunked(chunks, chrom_sizes: dict[str, int], bin_size: int 
         for st_ in streams:
             main.wait_stream(st_)
 
+    orders = [None] * len(chunks)
+
     def count(ci, status, stream):
         c0, fr = chunks[ci]
         level = C.c_int32(0)
+        orders[ci] = _cell_order(torch, L, fr.cell_ptr, fr.shape[0], stream)
         _lib.check(L.snapgpu_tile_count(fr.cell_ptr.data_ptr(), fr.pos.data_ptr(),
                                         _lib.ptr(fr.values) if fr.paired else 0, fr.shape[0],
                                         table.data_ptr(), n_chrom, int(bin_size), strategy,
                                         min_len, max_len, indptr.data_ptr() + 8 * (c0 + 1),
-                                        status.data_ptr(), C.byref(level), stream))
+                                        _lib.ptr(orders[ci]), status.data_ptr(), C.byref(level), stream))
         levels[ci] = level.value
 
     run(count)
@@ -200,7 +215,7 @@ def make_tile_matrix_chunked(chunks, chrom_sizes: dict[str, int], bin_size: int 
                                        table.data_ptr(), n_chrom, int(bin_size), strategy,
                                        min_len, max_len, levels[ci], indptr.data_ptr() + 8 * c0,
                                        out_idx.data_ptr(), idx_bits, out_val.data_ptr(),
-                                       status.data_ptr(), stream))
+                                       _lib.ptr(orders[ci]), status.data_ptr(), stream))
 
     run(fill)
     kept = {c: chrom_sizes[c] for c in names if c not in excl}
