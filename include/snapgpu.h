@@ -350,6 +350,81 @@ int snapgpu_gemm_nn_f64(const double* d_A, int64_t lda, const double* d_B, int64
 /* Seeded uniform(-1/2, 1/2) values (Lanczos start block). */
 int snapgpu_random_f64(uint64_t seed, int64_t count, double* d_out, void* stream);
 
+/* ---- Fragment-file import --------------------------------------------------
+ * Replaces internal.import_fragments (snapatac2-python/src/preprocessing.rs:87-171)
+ * and its core, preprocessing::import_fragments / make_arraydata /
+ * count_fragments (snapatac2-core/src/preprocessing/import.rs:31-204): the
+ * text of a fragment file ("chrom\tstart\tend\tbarcode[\tcount[\tstrand]]"
+ * lines, '#' comment lines skipped; Fragment::from_str, qc.rs:306-345),
+ * already decompressed and on the device (16-byte aligned), becomes
+ * obsm['fragment_paired'] (uint32 lengths) or obsm['fragment_single'] (int32
+ * signed lengths) as a CSR with int64 row pointers and uint32 genome base
+ * positions, plus the FragmentQC counts of every stored barcode (qc.rs:427-466).
+ * Rows: barcodes in file order (input sorted by barcode) or in byte order
+ * (unsorted input: the order of the reference's external sort), restricted to
+ * the whitelist and to at least min_num_fragments unique fragments; entries
+ * ascend by (position, value) inside a row.
+ * Sequence: snapgpu_frag_table_pack (host) for the chromosomes and the
+ * optional whitelist -> copy both to the device -> snapgpu_frag_count_lines ->
+ * snapgpu_frag_index (workspace from snapgpu_frag_workspace_bytes) -> allocate
+ * n_rows + 1 row pointers, nnz positions and values -> snapgpu_frag_fill.
+ * Records the reference would panic on fail with SNAPGPU_ERR_DATA and a
+ * message naming the 1-based line. */
+typedef struct snapgpu_frag_options {
+  int64_t shift_left;          /* added to start (and to end of single-end records) */
+  int64_t shift_right;         /* added to end of paired-end records */
+  uint64_t min_num_fragments;  /* rows need at least this many unique fragments */
+  int32_t sorted_by_barcode;   /* 1: groups are runs of equal barcodes in file order */
+  int32_t reserved;
+} snapgpu_frag_options;
+
+typedef struct snapgpu_frag_summary {
+  int64_t n_lines;    /* lines of the text */
+  int64_t n_records;  /* non-comment, non-empty lines */
+  int64_t n_kept;     /* records with end != start */
+  int64_t n_groups;   /* barcodes among them */
+  int64_t n_rows;     /* barcodes stored */
+  int64_t nnz;        /* fragments stored */
+  int32_t paired;     /* 1: fragment_paired, 0: fragment_single */
+  int32_t name_width; /* bytes of the longest barcode */
+} snapgpu_frag_summary;
+
+/* Host-only. A hashed name table: genome = 1 for the chromosomes (names in the
+ * stored order of uns['reference_sequences'], lengths, mito[k] = 1 for the
+ * mitochondrial names; the genome must be shorter than 2^32 bases), genome = 0
+ * for a barcode whitelist (lengths and mito may be NULL). */
+int64_t snapgpu_frag_table_bytes(const char* const* names, int32_t n);
+int snapgpu_frag_table_pack(const char* const* names, const uint64_t* lengths, const uint8_t* mito, int32_t n,
+                            int32_t genome, void* h_out, int64_t out_bytes);
+/* Number of lines of the text (a last line without '\n' counts). Synchronises. */
+int64_t snapgpu_frag_count_workspace_bytes(int64_t n_bytes);
+int snapgpu_frag_count_lines(const uint8_t* d_text, int64_t n_bytes, void* d_ws, int64_t ws_bytes,
+                             int64_t* h_n_lines, void* stream);
+/* Parse, group, QC and filter; the workspace keeps the state snapgpu_frag_fill
+ * reads. d_whitelist may be NULL. Synchronises. */
+int64_t snapgpu_frag_workspace_bytes(int64_t n_bytes, int64_t n_lines);
+int snapgpu_frag_index(const uint8_t* d_text, int64_t n_bytes, int64_t n_lines, const void* d_chroms,
+                       const void* d_whitelist, const snapgpu_frag_options* opt, void* d_ws, int64_t ws_bytes,
+                       snapgpu_frag_summary* h_summary, void* stream);
+/* d_cell_ptr[n_rows + 1], d_pos[nnz], d_val[nnz]; d_qc[3 n_rows] (optional):
+ * per row n_fragment (unique), total count, mitochondrial; d_names[n_rows x
+ * name_width] (optional): barcodes, zero padded. */
+int snapgpu_frag_fill(const uint8_t* d_text, int64_t n_bytes, const void* d_ws, int64_t ws_bytes,
+                      const snapgpu_frag_summary* h_summary, int64_t* d_cell_ptr, uint32_t* d_pos,
+                      uint32_t* d_val, uint64_t* d_qc, uint8_t* d_names, int32_t name_width, void* stream);
+
+/* Export: fragments -> text lines "chrom\tstart\tend\tbarcode\t1[\tstrand]\n",
+ * as export_fragments writes them (snapatac2-core/src/export.rs:90-106, with
+ * Fragment's Display, qc.rs:286-303). Line k prints fragment d_order[k] (or k
+ * if d_order is NULL); d_names[n_cells x name_width] holds the barcodes. Call
+ * once with d_text = NULL (line ends into d_line_end[n_fragments], total bytes
+ * to *h_text_bytes), then with the text buffer. Synchronises. */
+int64_t snapgpu_frag_format_workspace_bytes(int64_t n_fragments);
+int snapgpu_frag_format(const int64_t* d_cell_ptr, int64_t n_cells, const uint32_t* d_pos, const uint32_t* d_val,
+                        int64_t n_fragments, int32_t paired, const void* d_chroms, const uint8_t* d_names,
+                        int32_t name_width, const int64_t* d_order, int64_t* d_line_end, uint8_t* d_text,
+                        int64_t text_bytes, void* d_ws, int64_t ws_bytes, int64_t* h_text_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

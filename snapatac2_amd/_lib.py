@@ -70,6 +70,16 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_gemm_nn_f64": (C.c_int, [_p, _i64, _p, _i64, _i32, _i32, C.c_double, C.c_double,
                                       _p, _i64, _p]),
     "snapgpu_random_f64": (C.c_int, [_u64, _i64, _p, _p]),
+    "snapgpu_frag_table_bytes": (_i64, [C.POINTER(C.c_char_p), _i32]),
+    "snapgpu_frag_table_pack": (C.c_int, [C.POINTER(C.c_char_p), _p, _p, _i32, _i32, _p, _i64]),
+    "snapgpu_frag_count_workspace_bytes": (_i64, [_i64]),
+    "snapgpu_frag_count_lines": (C.c_int, [_p, _i64, _p, _i64, C.POINTER(_i64), _p]),
+    "snapgpu_frag_workspace_bytes": (_i64, [_i64, _i64]),
+    "snapgpu_frag_index": (C.c_int, [_p, _i64, _i64, _p, _p, _p, _p, _i64, _p, _p]),
+    "snapgpu_frag_fill": (C.c_int, [_p, _i64, _p, _i64, _p, _p, _p, _p, _p, _p, _i32, _p]),
+    "snapgpu_frag_format_workspace_bytes": (_i64, [_i64]),
+    "snapgpu_frag_format": (C.c_int, [_p, _i64, _p, _p, _i64, _i32, _p, _p, _i32, _p, _p, _p, _i64,
+                                      _p, _i64, C.POINTER(_i64), _p]),
 }
 
 
@@ -79,6 +89,18 @@ class PanelLayout(C.Structure):
                 ("chunk_panel", _p), ("n_chunks", _i64), ("row_pair_ptr", _p),
                 ("row_pair_ids", _p), ("n_pairs", _i64), ("n_rows", _i64), ("dense_rows", _i64),
                 ("packed", _i32), ("value_type", _i32)]
+
+
+class FragOptions(C.Structure):
+    """snapgpu_frag_options (include/snapgpu.h)."""
+    _fields_ = [("shift_left", _i64), ("shift_right", _i64), ("min_num_fragments", _u64),
+                ("sorted_by_barcode", _i32), ("reserved", _i32)]
+
+
+class FragSummary(C.Structure):
+    """snapgpu_frag_summary (include/snapgpu.h)."""
+    _fields_ = [("n_lines", _i64), ("n_records", _i64), ("n_kept", _i64), ("n_groups", _i64),
+                ("n_rows", _i64), ("nnz", _i64), ("paired", _i32), ("name_width", _i32)]
 
 
 class SnapGpuError(RuntimeError):

@@ -1,0 +1,207 @@
+"""Fragment-file import on the GPU against the oracle (bit-exact CSR, QC and
+barcodes), the reference's error cases, and two size-independent properties
+the reference's own tests use: export -> import gives the fragments back
+(test_import, snapatac2-python/tests/test_tools.py:128-142) and leaving
+chromosomes out at import equals excluding them at tile time (test_exclude,
+snapatac2-python/tests/test_pipeline.py:10-31)."""
+import numpy as np
+import pytest
+
+from oracle.fragments import OracleError, import_text
+
+pytestmark = pytest.mark.gpu
+
+GENOME = {"chr2": 3000, "chr1": 5000, "chrM": 300, "chrX": 2000}
+
+
+def _gpu(text, **kw):
+    from snapatac2_amd.pp import import_fragment_text
+    return import_fragment_text(text, GENOME if "chrom_sizes" not in kw else kw.pop("chrom_sizes"), **kw)
+
+
+def _check(ad, ref):
+    if len(ref["barcodes"]) == 0:
+        assert ad.n_obs == 0
+        return
+    key = "fragment_paired" if ref["paired"] else "fragment_single"
+    assert list(ad.obsm) == [key]
+    fr = ad.obsm[key]
+    assert list(ad.obs_names) == [b.decode() for b in ref["barcodes"]]
+    assert np.array_equal(fr.cell_ptr.cpu().numpy(), ref["indptr"])
+    assert np.array_equal(fr.pos.cpu().numpy().view(np.uint32), ref["pos"])
+    v = fr.values.cpu().numpy()
+    assert np.array_equal(v.view(np.uint32) if ref["paired"] else v, ref["val"])
+    assert np.array_equal(ad.obs["n_fragment"].to_numpy(), ref["n_fragment"])
+    assert np.array_equal(ad.obs["frac_dup"].to_numpy(), ref["frac_dup"], equal_nan=True)
+    assert np.array_equal(ad.obs["frac_mito"].to_numpy(), ref["frac_mito"], equal_nan=True)
+
+
+def _random_text(seed, n_lines, n_bc=40, sorted_bc=True, single=False, noise=True):
+    rng = np.random.default_rng(seed)
+    names = ["chr1", "chr2", "chrM", "chrX", "chrUn_x", "M"]
+    weights = np.asarray([40, 25, 5, 20, 5, 5], dtype=float)
+    bcs = [f"{''.join(rng.choice(list('ACGT'), 12))}-{rng.integers(1, 3)}" for _ in range(n_bc)]
+    rows = []
+    for _ in range(n_lines):
+        c = names[rng.choice(len(names), p=weights / weights.sum())]
+        ln = GENOME.get(c, 10_000)
+        size = min(int(rng.integers(0, 300)), ln // 2) if rng.random() > 0.03 else 0
+        s = int(rng.integers(0, max(ln - size - 10, 1)))  # room for shifts of a few bases
+        e = s + size
+        bc = bcs[min(int(rng.exponential(n_bc / 4)), n_bc - 1)]
+        f = [c, str(s), str(e), bc]
+        r = rng.random()
+        if single:
+            f += [str(rng.integers(1, 4)), "+" if rng.random() < 0.5 else "-"]
+        elif r < 0.3:
+            f += [str(rng.integers(0, 6))]
+        elif r < 0.4:
+            f += ["."]
+        elif r < 0.45:
+            f += [".", "."]
+        rows.append((bc, "\t".join(f)))
+    if sorted_bc:
+        rows.sort(key=lambda x: x[0])
+    out = []
+    for _, line in rows:
+        if noise and rng.random() < 0.01:
+            out.append("# a comment")
+        out.append(line + ("\r" if noise and rng.random() < 0.01 else ""))
+    return ("\n".join(out) + ("\n" if rng.random() < 0.5 else "")).encode()
+
+
+KNOWN = [
+    (b"# comment line\nchr1\t10\t20\tA\t1\nchr1\t5\t8\tA\t2\nchr2\t0\t4\tA\t1\nchrM\t1\t3\tA\t1\n"
+     b"chr3\t1\t3\tA\t1\nchr1\t7\t7\tA\t1\nchr1\t1\t2\tB\t3", dict(min_num_fragments=1)),
+    (b"chr1\t10\t15\tC\t1\t+\nchr1\t20\t26\tC\t1\t-\nchr1\t25\t30\tC\t1\t+\nchr2\t3\t9\tC\t1\t-\n",
+     dict(min_num_fragments=0)),
+    (b"chr1\t10\t20\tA\nchr1\t1\t9\tA\nchr1\t30\t40\tB\t1\t-\n",
+     dict(min_num_fragments=0, shift_left=-4, shift_right=-5, whitelist={"A"})),
+    (b"chr1\t1\t5\tb2\nchr1\t2\t5\tA\nchr1\t0\t5\tb2\nchr1\t3\t5\tB\nchr1\t9\t12\tA\n",
+     dict(min_num_fragments=0, sorted_by_barcode=False)),
+    (b"chr1\t1\t5\tA\t0\n", dict(min_num_fragments=0)),  # count 0: frac_dup = 1 - 1/0 = -inf
+    (b"", dict(min_num_fragments=0)),
+    (b"# only a comment\n\n", dict(min_num_fragments=0)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(KNOWN)))
+def test_known_cases(torch_gpu, case):
+    text, kw = KNOWN[case]
+    _check(_gpu(text, **kw), import_text(text, GENOME, **kw))
+
+
+@pytest.mark.parametrize("seed,n_lines,sorted_bc,single,kw", [
+    (0, 2_000, True, False, dict(min_num_fragments=0)),
+    (1, 30_000, True, False, dict(min_num_fragments=20)),
+    (2, 30_000, False, False, dict(min_num_fragments=5)),
+    (3, 20_000, True, True, dict(min_num_fragments=0)),
+    (4, 20_000, False, True, dict(min_num_fragments=3, shift_left=2)),
+    (5, 12_000, True, False, dict(min_num_fragments=0, shift_left=-4, shift_right=5)),
+    (6, 25_000, False, False, dict(min_num_fragments=0, whitelist="half")),
+    (7, 70_000, True, False, dict(min_num_fragments=0, n_bc=3000)),
+    (8, 70_000, False, False, dict(min_num_fragments=0, n_bc=3000)),
+])
+def test_random_files(torch_gpu, seed, n_lines, sorted_bc, single, kw):
+    kw = dict(kw)
+    n_bc = kw.pop("n_bc", 40)
+    text = _random_text(seed, n_lines, n_bc, sorted_bc, single)
+    if kw.get("whitelist") == "half":
+        bcs = sorted({ln.split(b"\t")[3] for ln in text.splitlines() if ln and not ln.startswith(b"#")})
+        kw["whitelist"] = {b.decode() for b in bcs[::2]}
+    ref = import_text(text, GENOME, sorted_by_barcode=sorted_bc, **kw)
+    assert len(ref["barcodes"]) > 0
+    _check(_gpu(text, sorted_by_barcode=sorted_bc, **kw), ref)
+
+
+ERRORS = [
+    (b"chr1\t1x\t5\tA\n", {}),
+    (b"chr1\n", {}),
+    (b"chr1\t1\n", {}),
+    (b"chr1\t1\t5\n", {}),
+    (b"chr1\t1\t5\tA\nchr1\t1\t5\tA\t1\t*\n", {}),
+    (b"chr1\t1\t5\tA\t-3\n", {}),
+    (b"chr1\t9\t5\tA\n", {}),
+    (b"chr1\t1\t5\t.\n", {}),
+    (b"chr1\t1\t5\tA\nchr2\t3000\t3010\tA\n", {}),
+    (b"chr1\t1\t5\tA\nchr1\t30\t40\tA\t1\t-\n", {}),
+    (b"chr1\t1\t5\tA\nchr1\t1\t5\tB\nchr1\t1\t5\tA\n", {}),
+    (b"chr1\t1\t5\tA\nchr1\t1\t5\tA\t1\t+\n", dict(sorted_by_barcode=False)),
+    (b"chr1\t1\t99999999999999999999999\tA\n", {}),
+]
+
+
+@pytest.mark.parametrize("case", range(len(ERRORS)))
+def test_errors(torch_gpu, case):
+    from snapatac2_amd._lib import PanicException
+    text, kw = ERRORS[case]
+    with pytest.raises(OracleError) as oe:
+        import_text(text, GENOME, min_num_fragments=0, **kw)
+    with pytest.raises(PanicException) as ge:
+        _gpu(text, min_num_fragments=0, **kw)
+    if oe.value.line is not None:
+        assert f"line {oe.value.line}:" in str(ge.value)
+
+
+def test_records_outside_whitelist_are_not_converted(torch_gpu):
+    """A '-' record in a paired file, or a start past its chromosome, fails only
+    when its barcode is whitelisted (count_fragments runs after the filter)."""
+    text = b"chr1\t1\t5\tA\nchr1\t30\t40\tB\t1\t-\nchr2\t4000\t4010\tC\n"
+    ad = _gpu(text, min_num_fragments=0, whitelist={"A"})
+    assert list(ad.obs_names) == ["A"]
+
+
+def _barcodes(n):
+    digits = np.asarray(list("ACGT"))
+    return ["".join(digits[(i >> (2 * k)) & 3] for k in range(9, -1, -1)) + "-1" for i in range(n)]
+
+
+def test_export_import_round_trip(torch_gpu):
+    torch = torch_gpu
+    from snapatac2_amd import synth
+    from snapatac2_amd.genome import synthetic_genome
+    from snapatac2_amd.pp import export_fragment_text, import_fragment_text
+    cs = synthetic_genome()
+    fr = synth.generate_chunk(synth.GeneratorTables(0), 0, 300)
+    bcs = _barcodes(300)  # byte order = row order
+    text, nb = export_fragment_text(fr, cs, bcs)
+    # the text matches the oracle's export of the same fragments
+    from oracle.fragments import export_lines
+    cp, pos, val = fr.cell_ptr.cpu().numpy(), fr.pos.cpu().numpy().view(np.uint32), \
+        fr.values.cpu().numpy().view(np.uint32)
+    head = text[:nb].cpu().numpy().tobytes().split(b"\n")[:2000]
+    want = export_lines(bcs, cp, pos, val, cs, True)[:2000]
+    assert [h.decode() for h in head] == want
+    for sorted_bc, order in [(True, None), (False, torch.randperm(fr.n_fragments, device="cuda"))]:
+        t, n = (text, nb) if order is None else export_fragment_text(fr, cs, bcs, order=order)
+        ad = import_fragment_text(t, cs, n_bytes=n, min_num_fragments=0, sorted_by_barcode=sorted_bc)
+        g = ad.obsm["fragment_paired"]
+        assert list(ad.obs_names) == bcs
+        assert torch.equal(g.cell_ptr, fr.cell_ptr)
+        assert torch.equal(g.pos, fr.pos)
+        assert torch.equal(g.values, fr.values)
+        counts = (fr.cell_ptr[1:] - fr.cell_ptr[:-1]).cpu().numpy()
+        assert np.array_equal(ad.obs["n_fragment"].to_numpy(), counts.astype(np.uint64))
+        assert np.all(ad.obs["frac_dup"].to_numpy() == 0.0)
+
+
+def test_exclude_at_import_equals_exclude_at_tiling(torch_gpu):
+    """test_exclude: chromosomes left out of chrom_sizes at import give the tile
+    matrix of excluding them at tile time."""
+    from snapatac2_amd import synth
+    from snapatac2_amd.genome import synthetic_genome
+    from snapatac2_amd.pp import export_fragment_text, import_fragment_text, make_tile_matrix
+    cs = synthetic_genome()
+    fr = synth.generate_chunk(synth.GeneratorTables(1), 0, 200)
+    bcs = _barcodes(200)
+    text, nb = export_fragment_text(fr, cs, bcs)
+    small = {k: v for k, v in cs.items() if k not in ("chr1", "chr10")}
+    a1 = import_fragment_text(text, small, n_bytes=nb, min_num_fragments=0, sorted_by_barcode=False)
+    a2 = import_fragment_text(text, cs, n_bytes=nb, min_num_fragments=0, sorted_by_barcode=False)
+    X1, n1 = make_tile_matrix(a1.obsm["fragment_paired"], small, 500, None)
+    X2, n2 = make_tile_matrix(a2.obsm["fragment_paired"], cs, 500, ["chr1", "chr10"])
+    X1, X2 = X1.to_scipy(), X2.to_scipy()
+    assert np.array_equal(X1.data, X2.data)
+    assert np.array_equal(X1.indices, X2.indices) and np.array_equal(X1.indptr, X2.indptr)
+    assert list(a1.obs_names) == list(a2.obs_names)
+    assert len(n1) == len(n2) and [n1[i] for i in (0, len(n1) - 1)] == [n2[i] for i in (0, len(n2) - 1)]
