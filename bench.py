@@ -10,7 +10,10 @@ rescoring). The plain path (IDF weights, no ascent) is timed once more on the
 same input and reported beside it.
 
 A step = one pass of the whole pipeline over the resident fragments. Prints one
-JSON line (rank 0). Per-stage times go to stderr.
+JSON line (rank 0). Per-stage times go to stderr. Beside it, "fragment_import"
+times the import of fragment text already resident in HBM (20,000 synthetic
+cells by default, as one barcode-sorted file and as one in random line order)
+against a one-thread C++ restatement on a 10,000-cell sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--cells N]
 """
@@ -227,6 +230,71 @@ def knn_precision(torch, n, k=50, d=30):
             "identical_graphs": same}
 
 
+def _barcodes(n):
+    """Fixed-width ACGT barcodes whose byte order is the row order."""
+    digits = np.asarray(list("ACGT"))
+    idx = np.arange(n)
+    cols = [digits[(idx >> (2 * k)) & 3] for k in range(11, -1, -1)]
+    return ["".join(t) + "-1" for t in zip(*cols)]
+
+
+def fragment_import(torch, n_cells, cpu_cells, reps=3):
+    """Fragment-file import: n_cells synthetic cells exported as fragment text
+    on the device (one line per fragment, rows in barcode order, as a file
+    sorted by barcode), then imported with the text resident in HBM; once as
+    is (sorted_by_barcode=True, the default) and once with the lines in a
+    random order (sorted_by_barcode=False). Both must give the fragments back.
+    CPU baseline: the one-thread C++ restatement on the text of the first
+    cpu_cells cells."""
+    from snapatac2_amd import synth
+    from snapatac2_amd._device import DeviceFragments
+    from snapatac2_amd.genome import synthetic_genome
+    from snapatac2_amd.pp import export_fragment_text, import_fragment_text
+    cs = synthetic_genome()
+    fr = synth.generate_chunk(synth.GeneratorTables(0), 0, n_cells)
+    bcs = _barcodes(n_cells)
+    out = {"workload": f"{n_cells} synthetic cells, {fr.n_fragments} fragment lines"}
+    for label, order in [("sorted", None), ("unsorted", "perm")]:
+        perm = None if order is None else torch.randperm(fr.n_fragments, device="cuda")
+        text, nb = export_fragment_text(fr, cs, bcs, order=perm)
+        del perm
+        ts = []
+        for _ in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ad = import_fragment_text(text, cs, n_bytes=nb, min_num_fragments=0,
+                                      sorted_by_barcode=order is None)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        g = ad.obsm["fragment_paired"]
+        ok = (list(ad.obs_names) == bcs and torch.equal(g.cell_ptr, fr.cell_ptr)
+              and torch.equal(g.pos, fr.pos) and torch.equal(g.values, fr.values))
+        dt = float(np.median(ts[1:]))
+        out[label] = {"s": round(dt, 4), "text_GB": round(nb / 1e9, 3),
+                      "GB_per_s": round(nb / dt / 1e9, 1),
+                      "fragments_per_s": round(fr.n_fragments / dt, 1), "round_trip": bool(ok)}
+        log(f"[bench] import ({label}): {dt * 1e3:.1f} ms for {nb / 1e9:.2f} GB, round trip {ok}")
+        del ad, g, text
+    if cpu_cells:
+        from oracle.fragments import import_text_cpp
+        m = min(cpu_cells, n_cells)
+        nf = int(fr.cell_ptr[m].item())
+        sub = DeviceFragments(fr.cell_ptr[:m + 1], fr.pos[:nf], fr.values[:nf], fr.n_cols, True)
+        text, nb = export_fragment_text(sub, cs, bcs[:m])
+        host = text[:nb].cpu().numpy().tobytes()
+        t0 = time.perf_counter()
+        ref = import_text_cpp(host, cs, min_num_fragments=0)
+        dt = time.perf_counter() - t0
+        ok = np.array_equal(ref["pos"], sub.pos.cpu().numpy().view(np.uint32))
+        out["cpu_baseline"] = {"value": round(nb / dt / 1e9, 3), "unit": "GB/s of text", "cores": 1,
+                               "kind": "port", "fragments_per_s": round(nf / dt, 1),
+                               "sample": f"text of the first {m} cells ({nf} lines, {nb / 1e9:.2f} GB), "
+                                         f"sorted by barcode", "s": round(dt, 2), "agrees": bool(ok)}
+        out["vs_cpu"] = round(out["sorted"]["GB_per_s"] / out["cpu_baseline"]["value"], 1)
+        log(f"[bench] import CPU baseline: {dt:.2f} s for {nb / 1e9:.2f} GB")
+    return out
+
+
 def cpu_baseline(torch, chunks, cfg, emb, sel_nnz_full):
     """Reference algorithm (oracle restatements) timed on this host's cores on a
     bounded sample, extrapolated to the workload's cell count."""
@@ -316,6 +384,8 @@ def main():
     ap.add_argument("--workload", default="pipeline-1m", choices=sorted(WORKLOADS))
     ap.add_argument("--cells", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--import-cells", type=int, default=20000,
+                    help="cells of the fragment-import measurement (0: skip it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one copy kernel of MIB MiB (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -397,6 +467,9 @@ def main():
     gceil = gather_ceiling(torch, roof, info["selected_nnz"])
     kprec = knn_precision(torch, n_cells)
     log(f"[bench] kNN precision: {kprec}")
+    fimp = None
+    if args.import_cells:
+        fimp = fragment_import(torch, args.import_cells, 0 if args.no_cpu_baseline else 10000)
     if gceil:
         log(f"[bench] gathers: operator {gceil['operator']} G/s, probe {gceil['probe']} G/s")
     cpu = None
@@ -423,6 +496,7 @@ def main():
         "roofline": roof,
         "gather_ceiling": gceil,
         "knn_precision": kprec,
+        "fragment_import": fimp,
         "cpu_baseline": cpu,
         "spmm_gbps": roof["achieved"] if roof else None,
         "knn_recall_sample": recall,

@@ -228,3 +228,96 @@ def export_lines(barcodes, indptr, pos, val, chrom_sizes: dict, paired: bool):
             else:
                 out.append(f"{names[c]}\t{max(coord + 1 + v, 0)}\t{coord + 1}\t{bc}\t1\t-")
     return out
+
+
+def import_text_cpp(text: bytes, chrom_sizes: dict, mito=("chrM", "M"), min_num_fragments: int = 200,
+                    sorted_by_barcode: bool = True, shift_left: int = 0, shift_right: int = 0,
+                    whitelist=None):
+    """The C++ restatement (oracle/import_oracle.cpp): same rules and result
+    layout as import_text, one thread; the timed CPU baseline of the import."""
+    import ctypes as C
+
+    from . import LIB_PATH, build
+    if not LIB_PATH.exists():
+        build()
+    h = C.CDLL(str(LIB_PATH))
+    h.oracle_import_run.restype = C.c_void_p
+    h.oracle_import_run.argtypes = [C.c_char_p, C.c_int64, C.POINTER(C.c_char_p), C.c_void_p, C.c_void_p,
+                                    C.c_int32, C.POINTER(C.c_char_p), C.c_int32, C.c_uint64, C.c_int32,
+                                    C.c_int64, C.c_int64]
+    for name, res in [("oracle_import_error", C.c_char_p), ("oracle_import_rows", C.c_int64),
+                      ("oracle_import_nnz", C.c_int64), ("oracle_import_paired", C.c_int32),
+                      ("oracle_import_width", C.c_int32)]:
+        getattr(h, name).restype = res
+        getattr(h, name).argtypes = [C.c_void_p]
+    h.oracle_import_copy.argtypes = [C.c_void_p] * 6 + [C.c_int32]
+    h.oracle_import_free.argtypes = [C.c_void_p]
+    names = [n.encode() for n in chrom_sizes]
+    lens = np.asarray([int(chrom_sizes[n]) for n in chrom_sizes], dtype=np.uint64)
+    mt = np.asarray([1 if n in set(mito) else 0 for n in chrom_sizes], dtype=np.uint8)
+    wl = [] if whitelist is None else [w.encode() if isinstance(w, str) else w for w in whitelist]
+    cn = (C.c_char_p * max(len(names), 1))(*names)
+    cw = (C.c_char_p * max(len(wl), 1))(*wl)
+    r = h.oracle_import_run(text, len(text), cn, lens.ctypes.data, mt.ctypes.data, len(names), cw,
+                            -1 if whitelist is None else len(wl), max(int(min_num_fragments), 0),
+                            1 if sorted_by_barcode else 0, int(shift_left), int(shift_right))
+    try:
+        e = h.oracle_import_error(r).decode()
+        if e:
+            parts = e.split()
+            raise OracleError(int(parts[1]), int(parts[3]) if len(parts) > 3 else None)
+        R, nnz, width = h.oracle_import_rows(r), h.oracle_import_nnz(r), h.oracle_import_width(r)
+        paired = bool(h.oracle_import_paired(r))
+        indptr = np.zeros(R + 1, dtype=np.int64)
+        pos = np.zeros(nnz, dtype=np.uint32)
+        val = np.zeros(nnz, dtype=np.uint32)
+        qc = np.zeros((R, 3), dtype=np.uint64)
+        nm = np.zeros((R, width), dtype=np.uint8)
+        h.oracle_import_copy(r, indptr.ctypes.data, pos.ctypes.data, val.ctypes.data, qc.ctypes.data,
+                             nm.ctypes.data, width)
+    finally:
+        h.oracle_import_free(r)
+    u, t, m = qc[:, 0], qc[:, 1], qc[:, 2]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        frac_dup = 1.0 - (u + m).astype(np.float64) / t.astype(np.float64)
+        frac_mito = m.astype(np.float64) / (u + m).astype(np.float64)
+    return dict(paired=paired, barcodes=[bytes(x).rstrip(b"\0") for x in nm], indptr=indptr, pos=pos,
+                val=val if paired else val.view(np.int32), n_fragment=u, frac_dup=frac_dup,
+                frac_mito=frac_mito)
+
+
+def random_text(genome: dict, seed: int, n_lines: int, n_bc: int = 40, sorted_bc: bool = True,
+                single: bool = False, noise: bool = True) -> bytes:
+    """A seeded fragment file over `genome` plus two unknown chromosome names:
+    empty fragments, optional count / strand fields, comments and CRLF lines."""
+    rng = np.random.default_rng(seed)
+    names = ["chr1", "chr2", "chrM", "chrX", "chrUn_x", "M"]
+    weights = np.asarray([40, 25, 5, 20, 5, 5], dtype=float)
+    bcs = [f"{''.join(rng.choice(list('ACGT'), 12))}-{rng.integers(1, 3)}" for _ in range(n_bc)]
+    rows = []
+    for _ in range(n_lines):
+        c = names[rng.choice(len(names), p=weights / weights.sum())]
+        ln = genome.get(c, 10_000)
+        size = min(int(rng.integers(0, 300)), ln // 2) if rng.random() > 0.03 else 0
+        s = int(rng.integers(0, max(ln - size - 10, 1)))  # room for shifts of a few bases
+        e = s + size
+        bc = bcs[min(int(rng.exponential(n_bc / 4)), n_bc - 1)]
+        f = [c, str(s), str(e), bc]
+        r = rng.random()
+        if single:
+            f += [str(rng.integers(1, 4)), "+" if rng.random() < 0.5 else "-"]
+        elif r < 0.3:
+            f += [str(rng.integers(0, 6))]
+        elif r < 0.4:
+            f += ["."]
+        elif r < 0.45:
+            f += [".", "."]
+        rows.append((bc, "\t".join(f)))
+    if sorted_bc:
+        rows.sort(key=lambda x: x[0])
+    out = []
+    for _, line in rows:
+        if noise and rng.random() < 0.01:
+            out.append("# a comment")
+        out.append(line + ("\r" if noise and rng.random() < 0.01 else ""))
+    return ("\n".join(out) + ("\n" if rng.random() < 0.5 else "")).encode()

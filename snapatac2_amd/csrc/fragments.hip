@@ -263,15 +263,20 @@ __device__ __forceinline__ uint64_t sat_add(uint64_t a, int64_t d) {
   return a < m ? 0 : a - m;
 }
 
-__global__ void __launch_bounds__(kThreads) parse_kernel(const uint8_t* __restrict__ text, int64_t n_bytes,
-                                                         const int64_t* __restrict__ line_start, int64_t n_lines,
-                                                         const uint8_t* __restrict__ chroms, int64_t shift_left,
-                                                         int64_t shift_right, ParseOut out, Counters* cnt) {
-  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  const int lane = threadIdx.x & 63;
+struct LineStats {
+  uint32_t n_rec, n_strand, n_keep, max_bc;
+  unsigned long long first;  // min (line << 1 | has_strand) over records
+};
+
+// Parses line i and writes its record; accumulates the block statistics.
+__device__ __forceinline__ void parse_line(const uint8_t* __restrict__ text, int64_t n_bytes,
+                                           const int64_t* __restrict__ line_start, int64_t i,
+                                           const uint8_t* __restrict__ chroms, int64_t shift_left,
+                                           int64_t shift_right, const ParseOut& out, Counters* cnt,
+                                           LineStats& ls) {
   bool is_rec = false, has_strand = false;
   uint32_t err = 0, bc_len_keep = 0;
-  if (i < n_lines) {
+  {
     const int64_t s = line_start[i];
     int64_t e = line_start[i + 1] - 1;
     if (e > n_bytes) e = n_bytes;
@@ -388,40 +393,64 @@ __global__ void __launch_bounds__(kThreads) parse_kernel(const uint8_t* __restri
     out.bc[i] = bcw;
     out.bch[i] = bch;
     if (err) atomicMin(&cnt->err, ((unsigned long long)i << 8) | err);
+    if (!err && (fl & F_KEEP)) ++ls.n_keep;
   }
-  const uint64_t rec = ballot64(is_rec), str = ballot64(is_rec && has_strand);
-  const int first = rec ? lowbit64(rec) : -1;
-  if (lane == 0 && rec) {
-    atomicAdd(&cnt->n_rec, (unsigned long long)popc64(rec));
-    if (str) atomicAdd(&cnt->n_strand, (unsigned long long)popc64(str));
+  if (is_rec) {
+    ++ls.n_rec;
+    ls.n_strand += has_strand ? 1 : 0;
+    const unsigned long long f = ((unsigned long long)i << 1) | (has_strand ? 1ull : 0ull);
+    ls.first = f < ls.first ? f : ls.first;
   }
-  if (first >= 0 && lane == first)
-    atomicMin(&cnt->first, ((unsigned long long)i << 1) | (has_strand ? 1ull : 0ull));
-  uint32_t mb = bc_len_keep;
+  ls.max_bc = bc_len_keep > ls.max_bc ? bc_len_keep : ls.max_bc;
+}
+
+// kLineBlock lines per block (lane t of round r takes line b0 + r * 256 + t);
+// blk[block] = records kept. The block statistics go to the counters with one
+// atomic each, so the counters see a few tens of thousands of atomics.
+__global__ void __launch_bounds__(kThreads) parse_kernel(const uint8_t* __restrict__ text, int64_t n_bytes,
+                                                         const int64_t* __restrict__ line_start, int64_t n_lines,
+                                                         const uint8_t* __restrict__ chroms, int64_t shift_left,
+                                                         int64_t shift_right, ParseOut out, Counters* cnt,
+                                                         int64_t* __restrict__ blk) {
+  __shared__ unsigned long long red[5][kThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int64_t b0 = (int64_t)blockIdx.x * kLineBlock;
+  LineStats ls{0, 0, 0, 0, ~0ull};
+  for (int r = 0; r < kRounds; ++r) {
+    const int64_t i = b0 + (int64_t)r * kThreads + threadIdx.x;
+    if (i < n_lines) parse_line(text, n_bytes, line_start, i, chroms, shift_left, shift_right, out, cnt, ls);
+  }
+  unsigned long long v[5] = {ls.n_rec, ls.n_strand, ls.n_keep, ls.max_bc, ls.first};
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) {
-    const uint32_t o = __shfl_xor(mb, d, 64);
-    mb = mb > o ? mb : o;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) v[q] += __shfl_xor(v[q], d, 64);
+    const unsigned long long m = __shfl_xor(v[3], d, 64), f = __shfl_xor(v[4], d, 64);
+    v[3] = m > v[3] ? m : v[3];
+    v[4] = f < v[4] ? f : v[4];
   }
-  if (lane == 0 && mb) atomicMax(&cnt->max_bc, mb);
+  if (lane == 0)
+    for (int q = 0; q < 5; ++q) red[q][wid] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / 64; ++w) {
+      for (int q = 0; q < 3; ++q) v[q] += red[q][w];
+      v[3] = red[3][w] > v[3] ? red[3][w] : v[3];
+      v[4] = red[4][w] < v[4] ? red[4][w] : v[4];
+    }
+    blk[blockIdx.x] = (int64_t)v[2];
+    if (v[0]) atomicAdd(&cnt->n_rec, v[0]);
+    if (v[1]) atomicAdd(&cnt->n_strand, v[1]);
+    if (v[3] && v[3] > *reinterpret_cast<volatile unsigned int*>(&cnt->max_bc))
+      atomicMax(&cnt->max_bc, (unsigned int)v[3]);
+    if (v[4] != ~0ull && v[4] < *reinterpret_cast<volatile unsigned long long*>(&cnt->first))
+      atomicMin(&cnt->first, v[4]);
+  }
 }
 
 // ---------------------------------------------------------------------------
 // 3. compaction of the kept records (file order)
 // ---------------------------------------------------------------------------
-
-__global__ void __launch_bounds__(kThreads) keep_count_kernel(const uint8_t* __restrict__ flags, int64_t n,
-                                                              int64_t* __restrict__ blk) {
-  __shared__ int64_t red[kThreads / 64];
-  const int64_t b0 = (int64_t)blockIdx.x * kLineBlock;
-  int64_t c = 0;
-  for (int r = 0; r < kRounds; ++r) {
-    const int64_t i = b0 + (int64_t)r * kThreads + threadIdx.x;
-    c += (i < n && (flags[i] & F_KEEP)) ? 1 : 0;
-  }
-  c = block_sum<int64_t>(c, red);
-  if (threadIdx.x == 0) blk[blockIdx.x] = c;
-}
 
 __global__ void __launch_bounds__(kThreads) keep_write_kernel(const uint8_t* __restrict__ flags, int64_t n,
                                                               const int64_t* __restrict__ blk_incl,
@@ -452,11 +481,31 @@ __global__ void __launch_bounds__(kThreads) keep_write_kernel(const uint8_t* __r
 // 4. sort keys and barcode comparisons
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ bool same_barcode(const uint8_t* text, const uint64_t* bc, const uint64_t* bch,
+// 8 bytes starting at byte p (any alignment) from two aligned loads.
+__device__ __forceinline__ uint64_t load_u8x8(const uint8_t* t, int64_t p, int64_t n) {
+  const int64_t a = p & ~(int64_t)7;
+  const int sh = (int)(p & 7) * 8;
+  const uint64_t lo = load_word(t, a, n);
+  if (sh == 0) return lo;
+  return (lo >> sh) | (load_word(t, a + 8, n) << (64 - sh));
+}
+
+// Equality of the len bytes at offsets a and b, eight at a time.
+__device__ __forceinline__ bool text_equal(const uint8_t* t, int64_t n, int64_t a, int64_t b, uint32_t len) {
+  for (uint32_t k = 0; k < len; k += 8) {
+    const uint64_t x = load_u8x8(t, a + k, n) ^ load_u8x8(t, b + k, n);
+    const uint32_t rem = len - k;
+    const uint64_t mask = rem >= 8 ? ~0ull : ((1ull << (8 * rem)) - 1);
+    if (x & mask) return false;
+  }
+  return true;
+}
+
+__device__ __forceinline__ bool same_barcode(const uint8_t* text, int64_t n, const uint64_t* bc, const uint64_t* bch,
                                              uint32_t la, uint32_t lb, bool* collision) {
   if (bch[la] != bch[lb]) return false;
   const uint64_t a = bc[la], b = bc[lb];
-  const bool eq = (a & 255) == (b & 255) && bytes_equal(text + (a >> 8), text + (b >> 8), (uint32_t)(a & 255));
+  const bool eq = (a & 255) == (b & 255) && text_equal(text, n, (int64_t)(a >> 8), (int64_t)(b >> 8), (uint32_t)(a & 255));
   if (!eq) *collision = true;
   return eq;
 }
@@ -488,13 +537,13 @@ __global__ void pv_key_kernel(const uint32_t* __restrict__ perm, const uint32_t*
 
 // Boundaries of equal-barcode runs over kept records in file order (input
 // sorted by barcode): run[v] = 1 where a new run starts.
-__global__ void run_bound_kernel(const uint8_t* __restrict__ text, const uint32_t* __restrict__ kept,
+__global__ void run_bound_kernel(const uint8_t* __restrict__ text, int64_t n_bytes, const uint32_t* __restrict__ kept,
                                  const uint64_t* __restrict__ bc, const uint64_t* __restrict__ bch, int64_t n,
                                  int64_t* __restrict__ run) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   bool col = false;
-  run[v] = (v == 0 || !same_barcode(text, bc, bch, kept[v - 1], kept[v], &col)) ? 1 : 0;
+  run[v] = (v == 0 || !same_barcode(text, n_bytes, bc, bch, kept[v - 1], kept[v], &col)) ? 1 : 0;
 }
 
 // Second sort key: the run index (sorted input) or the barcode hash.
@@ -507,16 +556,20 @@ __global__ void group_key_kernel(const uint32_t* __restrict__ perm, const uint32
   key[j] = run_id ? (uint64_t)run_id[v] : bch[kept[v]];
 }
 
-// Group boundaries over the final order; flags two different barcodes with
-// one hash next to each other (meaningful for the hash-sorted order).
-__global__ void group_bound_kernel(const uint8_t* __restrict__ text, const uint32_t* __restrict__ perm,
+// Group boundaries over the final order: run changes (sorted input; run ids
+// by kept record) or barcode changes (hash-sorted order, where two different
+// barcodes with one hash next to each other are flagged).
+__global__ void group_bound_kernel(const uint8_t* __restrict__ text, int64_t n_bytes, const uint32_t* __restrict__ perm,
                                    const uint32_t* __restrict__ kept, const uint64_t* __restrict__ bc,
-                                   const uint64_t* __restrict__ bch, int64_t n, int64_t* __restrict__ gid,
-                                   Counters* cnt) {
+                                   const uint64_t* __restrict__ bch, const int64_t* __restrict__ run, int64_t n,
+                                   int64_t* __restrict__ gid, Counters* cnt) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  bool col = false;
-  const bool b = j == 0 || !same_barcode(text, bc, bch, kept[perm[j - 1]], kept[perm[j]], &col);
+  bool b = j == 0, col = false;
+  if (!b) {
+    if (run) b = run[perm[j - 1]] != run[perm[j]];
+    else b = !same_barcode(text, n_bytes, bc, bch, kept[perm[j - 1]], kept[perm[j]], &col);
+  }
   gid[j] = b ? 1 : 0;
   if (col) atomicOr(&cnt->collision, 1u);
 }
@@ -1128,14 +1181,12 @@ int snapgpu_frag_index(const uint8_t* d_text, int64_t n_bytes, int64_t n_lines, 
   SG_HIP(hipMemcpyAsync(C, &c0, sizeof(Counters), hipMemcpyHostToDevice, st));
   ParseOut po{at<uint32_t>(w, Lo.pos), at<uint32_t>(w, Lo.size), at<uint32_t>(w, Lo.cnt), at<uint8_t>(w, Lo.flags),
               at<uint64_t>(w, Lo.bc), at<uint64_t>(w, Lo.bch)};
-  parse_kernel<<<grid(n_lines), kThreads, 0, st>>>(d_text, n_bytes, line_start, n_lines, chroms, opt->shift_left,
-                                                   opt->shift_right, po, C);
+  const int64_t lb = ceil_div(n_lines, kLineBlock);
+  parse_kernel<<<(unsigned)lb, kThreads, 0, st>>>(d_text, n_bytes, line_start, n_lines, chroms, opt->shift_left,
+                                                  opt->shift_right, po, C, blk);
   SG_LAUNCH_CHECK();
 
   // 3. compaction
-  const int64_t lb = ceil_div(n_lines, kLineBlock);
-  keep_count_kernel<<<(unsigned)lb, kThreads, 0, st>>>(po.flags, n_lines, blk);
-  SG_LAUNCH_CHECK();
   rc = snapgpu_scan_inclusive_i64(blk, lb, sws, sws_bytes, stream);
   if (rc) return rc;
   uint32_t* kept = at<uint32_t>(w, Lo.kept);
@@ -1185,7 +1236,7 @@ int snapgpu_frag_index(const uint8_t* d_text, int64_t n_bytes, int64_t n_lines, 
   int64_t* run = at<int64_t>(w, Lo.run);
   int end_bit = 64;
   if (opt->sorted_by_barcode) {
-    run_bound_kernel<<<grid(K), kThreads, 0, st>>>(d_text, kept, po.bc, po.bch, K, run);
+    run_bound_kernel<<<grid(K), kThreads, 0, st>>>(d_text, n_bytes, kept, po.bc, po.bch, K, run);
     SG_LAUNCH_CHECK();
     rc = snapgpu_scan_inclusive_i64(run, K, sws, sws_bytes, stream);
     if (rc) return rc;
@@ -1207,7 +1258,8 @@ int snapgpu_frag_index(const uint8_t* d_text, int64_t n_bytes, int64_t n_lines, 
 
   // 5. groups
   int64_t* gid = at<int64_t>(w, Lo.gid);
-  group_bound_kernel<<<grid(K), kThreads, 0, st>>>(d_text, perm, kept, po.bc, po.bch, K, gid, C);
+  group_bound_kernel<<<grid(K), kThreads, 0, st>>>(d_text, n_bytes, perm, kept, po.bc, po.bch,
+                                                   opt->sorted_by_barcode ? run : nullptr, K, gid, C);
   SG_LAUNCH_CHECK();
   rc = snapgpu_scan_inclusive_i64(gid, K, sws, sws_bytes, stream);
   if (rc) return rc;

@@ -7,7 +7,7 @@ snapatac2-python/tests/test_pipeline.py:10-31)."""
 import numpy as np
 import pytest
 
-from oracle.fragments import OracleError, import_text
+from oracle.fragments import OracleError, import_text, random_text
 
 pytestmark = pytest.mark.gpu
 
@@ -36,38 +36,8 @@ def _check(ad, ref):
     assert np.array_equal(ad.obs["frac_mito"].to_numpy(), ref["frac_mito"], equal_nan=True)
 
 
-def _random_text(seed, n_lines, n_bc=40, sorted_bc=True, single=False, noise=True):
-    rng = np.random.default_rng(seed)
-    names = ["chr1", "chr2", "chrM", "chrX", "chrUn_x", "M"]
-    weights = np.asarray([40, 25, 5, 20, 5, 5], dtype=float)
-    bcs = [f"{''.join(rng.choice(list('ACGT'), 12))}-{rng.integers(1, 3)}" for _ in range(n_bc)]
-    rows = []
-    for _ in range(n_lines):
-        c = names[rng.choice(len(names), p=weights / weights.sum())]
-        ln = GENOME.get(c, 10_000)
-        size = min(int(rng.integers(0, 300)), ln // 2) if rng.random() > 0.03 else 0
-        s = int(rng.integers(0, max(ln - size - 10, 1)))  # room for shifts of a few bases
-        e = s + size
-        bc = bcs[min(int(rng.exponential(n_bc / 4)), n_bc - 1)]
-        f = [c, str(s), str(e), bc]
-        r = rng.random()
-        if single:
-            f += [str(rng.integers(1, 4)), "+" if rng.random() < 0.5 else "-"]
-        elif r < 0.3:
-            f += [str(rng.integers(0, 6))]
-        elif r < 0.4:
-            f += ["."]
-        elif r < 0.45:
-            f += [".", "."]
-        rows.append((bc, "\t".join(f)))
-    if sorted_bc:
-        rows.sort(key=lambda x: x[0])
-    out = []
-    for _, line in rows:
-        if noise and rng.random() < 0.01:
-            out.append("# a comment")
-        out.append(line + ("\r" if noise and rng.random() < 0.01 else ""))
-    return ("\n".join(out) + ("\n" if rng.random() < 0.5 else "")).encode()
+def _random_text(seed, n_lines, n_bc=40, sorted_bc=True, single=False):
+    return random_text(GENOME, seed, n_lines, n_bc, sorted_bc, single)
 
 
 KNOWN = [

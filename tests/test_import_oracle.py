@@ -111,3 +111,36 @@ def test_export_round_trip():
     out = export_lines([b.decode() for b in r["barcodes"]], r["indptr"], r["pos"], r["val"], GENOME,
                        r["paired"])
     assert sorted(out) == sorted(lines)
+
+
+GENOME2 = {"chr2": 3000, "chr1": 5000, "chrM": 300, "chrX": 2000}
+
+
+@pytest.mark.parametrize("seed,sorted_bc,single,kw", [
+    (0, True, False, dict(min_num_fragments=0)),
+    (1, False, False, dict(min_num_fragments=5)),
+    (2, True, True, dict(min_num_fragments=0, shift_left=-3)),
+    (3, False, True, dict(min_num_fragments=2, whitelist=None)),
+    (4, True, False, dict(min_num_fragments=0, shift_left=-4, shift_right=5)),
+])
+def test_cpp_restatement_matches(seed, sorted_bc, single, kw):
+    """The C++ restatement (the timed CPU baseline) gives the Python
+    restatement's result on seeded files."""
+    from oracle.fragments import import_text_cpp, random_text
+    text = random_text(GENOME2, seed, 5000, 60, sorted_bc, single)
+    a = import_text(text, GENOME2, sorted_by_barcode=sorted_bc, **kw)
+    b = import_text_cpp(text, GENOME2, sorted_by_barcode=sorted_bc, **kw)
+    assert a["paired"] == b["paired"] and a["barcodes"] == b["barcodes"]
+    for k in ("indptr", "pos", "val", "n_fragment"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("frac_dup", "frac_mito"):
+        assert np.array_equal(a[k], b[k], equal_nan=True), k
+
+
+def test_cpp_restatement_errors():
+    from oracle.fragments import import_text_cpp
+    for text, code, line in [(b"chr1\t1x\t5\tA\n", E_START_BAD, 1),
+                             (b"chr1\t1\t5\tA\nchr2\t50\t60\tA\n", E_RANGE, 2)]:
+        with pytest.raises(OracleError) as e:
+            import_text_cpp(text, GENOME, min_num_fragments=0)
+        assert (e.value.code, e.value.line) == (code, line)
