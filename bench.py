@@ -386,6 +386,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--import-cells", type=int, default=20000,
                     help="cells of the fragment-import measurement (0: skip it)")
+    ap.add_argument("--import-only", action="store_true",
+                    help="run only the fragment-import measurement (its own JSON line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one copy kernel of MIB MiB (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -405,6 +407,10 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] copy calibration: {nb} bytes read, {nb} bytes written")
         del src, dst
+    if args.import_only:
+        print(json.dumps({"fragment_import": fragment_import(
+            torch, args.import_cells or 20000, 0 if args.no_cpu_baseline else 10000)}), flush=True)
+        return
     cfg = list(WORKLOADS[args.workload])
     if args.cells:
         cfg[0] = args.cells

@@ -95,7 +95,8 @@ struct Counters {
   unsigned long long dup;       // min group index of a barcode seen in two runs
   unsigned int max_bc;          // longest barcode (bytes) over kept records
   unsigned int collision;       // two barcodes with one hash next to each other
-  unsigned int pad[2];
+  unsigned int any_bad;         // a whitelisted group holds a record count_fragments cannot convert
+  unsigned int pad;
 };
 
 __host__ __device__ inline uint64_t fnv_step(uint64_t h, uint32_t c) { return (h ^ c) * kFnvPrime; }
@@ -582,12 +583,14 @@ __global__ void __launch_bounds__(kThreads) group_stats_kernel(
     const uint32_t* __restrict__ perm, const uint32_t* __restrict__ kept, const uint8_t* __restrict__ flags,
     const uint32_t* __restrict__ cnt_in, const int64_t* __restrict__ gid, int64_t n,
     int64_t* __restrict__ gstart, uint32_t* __restrict__ gline, unsigned long long* __restrict__ g_total,
-    unsigned long long* __restrict__ g_mito, unsigned long long* __restrict__ g_uniq) {
+    unsigned long long* __restrict__ g_mito, unsigned long long* __restrict__ g_uniq,
+    unsigned int* __restrict__ g_bad) {
   const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const bool act = j < n;
   int64_t g = -1;
   uint64_t tot = 0, mito = 0, uniq = 0;
+  uint32_t bad = 0;  // conversion flags of in-genome records
   if (act) {
     const uint32_t line = kept[perm[j]];
     g = gid[j] - 1;
@@ -596,6 +599,7 @@ __global__ void __launch_bounds__(kThreads) group_stats_kernel(
       tot = cnt_in[line];
       mito = (f & F_MITO) ? 1 : 0;
       uniq = 1 - mito;
+      bad = f & (F_RANGE | F_HUGE | F_REV | F_BIG);
     }
     if (j == 0 || gid[j - 1] != gid[j]) {
       gstart[g] = j;
@@ -607,10 +611,12 @@ __global__ void __launch_bounds__(kThreads) group_stats_kernel(
     const int src = lane >= d ? lane - d : lane;
     const int64_t og = (int64_t)shfl_u64((uint64_t)g, src);
     const uint64_t ot = shfl_u64(tot, src), om = shfl_u64(mito, src), ou = shfl_u64(uniq, src);
+    const uint32_t ob = __shfl(bad, src, 64);
     if (lane >= d && og == g) {
       tot += ot;
       mito += om;
       uniq += ou;
+      bad |= ob;
     }
   }
   const int64_t ng = (int64_t)shfl_u64((uint64_t)g, lane < 63 ? lane + 1 : lane);
@@ -619,7 +625,18 @@ __global__ void __launch_bounds__(kThreads) group_stats_kernel(
     if (tot) atomicAdd(&g_total[g], (unsigned long long)tot);
     if (mito) atomicAdd(&g_mito[g], (unsigned long long)mito);
     if (uniq) atomicAdd(&g_uniq[g], (unsigned long long)uniq);
+    if (bad) atomicOr(&g_bad[g], bad);
   }
+}
+
+// Whether a whitelisted group holds a record that count_fragments cannot
+// convert (then bad_kernel names the first such line).
+__global__ void bad_group_kernel(const unsigned int* __restrict__ g_bad, const uint8_t* __restrict__ wl,
+                                 int64_t n_groups, int paired, Counters* cnt) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_groups) return;
+  const uint32_t mask = F_RANGE | F_HUGE | (paired ? F_REV : F_BIG);
+  if (wl[g] && (g_bad[g] & mask)) atomicOr(&cnt->any_bad, 1u);
 }
 
 // Whitelist membership of each group's barcode (import.rs:74).
@@ -927,7 +944,7 @@ __global__ void fmt_write_kernel(const int64_t* __restrict__ cell_ptr, int64_t n
 
 struct Layout {
   int64_t blk, scan, line_start, pos, size, cnt, flags, bc, bch, kept, permA, permB, keyA, keyB, gid, run,
-      gstart, gline, gtot, gmito, guniq, wl, ordA, ordB, gkA, gkB, kr, rowid, rowgroup, rowlen, counters, tmp,
+      gstart, gline, gtot, gmito, guniq, gbad, wl, ordA, ordB, gkA, gkB, kr, rowid, rowgroup, rowlen, counters, tmp,
       total;
   size_t tmp_bytes;
 };
@@ -969,6 +986,7 @@ Layout make_layout(int64_t n_bytes, int64_t L) {
   o.gtot = take(8 * Lc);
   o.gmito = take(8 * Lc);
   o.guniq = take(8 * Lc);
+  o.gbad = take(4 * Lc);
   o.wl = take(Lc);
   o.ordA = take(4 * Lc);
   o.ordB = take(4 * Lc);
@@ -1280,8 +1298,10 @@ int snapgpu_frag_index(const uint8_t* d_text, int64_t n_bytes, int64_t n_lines, 
   SG_HIP(hipMemsetAsync(gtot, 0, 8 * G, st));
   SG_HIP(hipMemsetAsync(gmito, 0, 8 * G, st));
   SG_HIP(hipMemsetAsync(guniq, 0, 8 * G, st));
+  auto* gbad = at<unsigned int>(w, Lo.gbad);
+  SG_HIP(hipMemsetAsync(gbad, 0, 4 * G, st));
   group_stats_kernel<<<grid(K), kThreads, 0, st>>>(perm, kept, po.flags, po.cnt, gid, K, gstart, gline, gtot, gmito,
-                                                   guniq);
+                                                   guniq, gbad);
   SG_LAUNCH_CHECK();
   uint8_t* wl = at<uint8_t>(w, Lo.wl);
   whitelist_kernel<<<grid(G), kThreads, 0, st>>>(d_text, gline, po.bc, po.bch, G,
@@ -1300,10 +1320,16 @@ int snapgpu_frag_index(const uint8_t* d_text, int64_t n_bytes, int64_t n_lines, 
     dup_kernel<<<grid(G), kThreads, 0, st>>>(d_text, gk.Current(), go.Current(), gline, po.bc, wl, G, C);
     SG_LAUNCH_CHECK();
   }
-  bad_kernel<<<grid(K), kThreads, 0, st>>>(perm, kept, po.flags, gid, wl, K, paired, C);
+  bad_group_kernel<<<grid(G), kThreads, 0, st>>>(gbad, wl, G, paired, C);
   SG_LAUNCH_CHECK();
   SG_HIP(hipMemcpyAsync(&ch, C, sizeof(Counters), hipMemcpyDeviceToHost, st));
   SG_HIP(hipStreamSynchronize(st));
+  if (ch.any_bad) {
+    bad_kernel<<<grid(K), kThreads, 0, st>>>(perm, kept, po.flags, gid, wl, K, paired, C);
+    SG_LAUNCH_CHECK();
+    SG_HIP(hipMemcpyAsync(&ch, C, sizeof(Counters), hipMemcpyDeviceToHost, st));
+    SG_HIP(hipStreamSynchronize(st));
+  }
   if (ch.dup != ~0ull) {
     set_error("a barcode appears in two separate runs: please sort the fragment file by barcodes (group " +
               std::to_string(ch.dup) + ")");
