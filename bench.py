@@ -295,6 +295,48 @@ def fragment_import(torch, n_cells, cpu_cells, reps=3):
     return out
 
 
+def nystrom_bench(torch, Xs, n_comps, sample_size, cpu_cells):
+    """Nystrom spectral (spectral(sample_size=...), the path for cell counts
+    beyond one operator): landmarks, their eigenpairs, the two sparse products
+    over all cells, the per-chunk normalisation and orthogonalize, on the
+    selected matrix of the pipeline. CPU baseline: the oracle (scipy) on the
+    first cpu_cells cells with the same landmark fraction."""
+    from snapatac2_amd.tl._nystrom import landmarks, orthogonalize, spectral_embedding_nystrom
+    n = Xs.shape[0]
+    s = min(sample_size, n)
+    ts = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        v, q = spectral_embedding_nystrom(Xs, n_comps, s)
+        ev, U = orthogonalize(v, q)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    dt = ts[-1]
+    out = {"workload": f"{n} cells x {Xs.shape[1]} features, {s} landmarks, {n_comps} components",
+           "s": round(dt, 3), "cells_per_s": round(n / dt, 1), "leading_eigenvalue": float(ev[0])}
+    log(f"[bench] Nystrom: {dt:.3f}s for {n} cells ({s} landmarks)")
+    if cpu_cells:
+        from oracle import spectral as osp
+        m = min(cpu_cells, n)
+        sub = Xs.indptr[:m + 1]
+        nnz = int(sub[-1].item())
+        import scipy.sparse as sp
+        Xh = sp.csr_matrix((Xs.data[:nnz].cpu().numpy().view(np.uint32), Xs.indices[:nnz].cpu().numpy(),
+                            sub.cpu().numpy()), shape=(m, Xs.shape[1]))
+        rows = landmarks(m, max(2, int(round(m * s / n))))
+        t0 = time.perf_counter()
+        cv, cq = osp.nystrom(Xh, rows, n_comps)
+        osp.orthogonalize(cv, cq)
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(m / cdt, 1), "unit": "cells/s", "cores": 16, "kind": "port",
+                               "sample": f"first {m} cells, {rows.size} landmarks (scipy eigsh + sparse "
+                                         f"products)", "s": round(cdt, 2)}
+        out["vs_cpu"] = round(out["cells_per_s"] / out["cpu_baseline"]["value"], 1)
+        log(f"[bench] Nystrom CPU baseline: {cdt:.2f}s for {m} cells")
+    return out
+
+
 def cpu_baseline(torch, chunks, cfg, emb, sel_nnz_full):
     """Reference algorithm (oracle restatements) timed on this host's cores on a
     bounded sample, extrapolated to the workload's cell count."""
@@ -386,6 +428,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--import-cells", type=int, default=20000,
                     help="cells of the fragment-import measurement (0: skip it)")
+    ap.add_argument("--nystrom-landmarks", type=int, default=100_000,
+                    help="landmarks of the Nystrom measurement on the pipeline's matrix (0: skip it)")
     ap.add_argument("--import-only", action="store_true",
                     help="run only the fragment-import measurement (its own JSON line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -448,6 +492,7 @@ def main():
     value = n_cells / (ms / 1e3)
     stages = {k: round(v, 3) for k, v in timer.t.items()}
     plain = None
+    nys_X = info["op"].X if "op" in info else None
     if cfg[5] > 0:
         # the plain GPU path (IDF weights, no ascent) on the same resident input
         ptimer = StageTimer(torch)
@@ -457,6 +502,7 @@ def main():
         pinfo, _, _ = run_pipeline(torch, chunks, cfg, ptimer, None, learn_steps=0)
         torch.cuda.synchronize()
         pdt = time.perf_counter() - t0
+        nys_X = pinfo["op"].X  # the selected matrix, for the Nystrom measurement
         del pinfo["op"]
         plain = {"ms_per_step": round(pdt * 1e3, 1), "cells_per_s": round(n_cells / pdt, 1),
                  "stages_s": {k: round(v, 3) for k, v in ptimer.t.items()},
@@ -473,6 +519,11 @@ def main():
     gceil = gather_ceiling(torch, roof, info["selected_nnz"])
     kprec = knn_precision(torch, n_cells)
     log(f"[bench] kNN precision: {kprec}")
+    nys = None
+    if args.nystrom_landmarks and nys_X is not None:
+        nys = nystrom_bench(torch, nys_X, cfg[2], args.nystrom_landmarks,
+                            0 if args.no_cpu_baseline else 20000)
+    del nys_X
     fimp = None
     if args.import_cells:
         fimp = fragment_import(torch, args.import_cells, 0 if args.no_cpu_baseline else 10000)
@@ -503,6 +554,7 @@ def main():
         "gather_ceiling": gceil,
         "knn_precision": kprec,
         "fragment_import": fimp,
+        "nystrom": nys,
         "cpu_baseline": cpu,
         "spmm_gbps": roof["achieved"] if roof else None,
         "knn_recall_sample": recall,

@@ -425,6 +425,39 @@ int snapgpu_frag_format(const int64_t* d_cell_ptr, int64_t n_cells, const uint32
                         int32_t name_width, const int64_t* d_order, int64_t* d_line_end, uint8_t* d_text,
                         int64_t text_bytes, void* d_ws, int64_t ws_bytes, int64_t* h_text_bytes, void* stream);
 
+/* ---- Nystrom spectral embedding ----------------------------------------------
+ * Replaces internal.spectral_embedding_nystrom (snapatac2-python/src/embedding.rs:60-131)
+ * with nystrom (:193-268) and idf_from_chunks (:288-312). The host picks the
+ * landmark cells; the landmark embedding is the ordinary spectral path above.
+ * Rows d_rows[n_take] of a CSR (int32 columns, uint32 values): row pointers
+ * d_out_indptr[n_take + 1] and *h_nnz first (synchronises), then the entries. */
+int64_t snapgpu_csr_take_rows_workspace_bytes(int64_t n_take);
+int snapgpu_csr_take_rows_count(const int64_t* d_indptr, int64_t n_rows, const int64_t* d_rows, int64_t n_take,
+                                int64_t* d_out_indptr, void* d_ws, int64_t ws_bytes, int64_t* h_nnz, void* stream);
+int snapgpu_csr_take_rows_fill(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_val,
+                               const int64_t* d_rows, int64_t n_take, const int64_t* d_out_indptr,
+                               int32_t* d_out_idx, uint32_t* d_out_val, void* stream);
+/* Entries per column (document frequency of idf_from_chunks), uint64 d_out[m];
+ * d_ws16: 16 bytes of device scratch. */
+int snapgpu_csr_col_counts(const int64_t* d_indptr, int64_t n, const int32_t* d_idx, int64_t nnz, int64_t m,
+                           uint64_t* d_out, int32_t* d_ws16, void* stream);
+/* d_s[i] = 1 / ||x_i o w||_2, 0 for an empty row (normalize, embedding.rs:314-326). */
+int snapgpu_csr_row_inv_norm(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_cnt, int64_t n,
+                             const double* d_w, double* d_s, void* stream);
+/* C[i, :] = row_scale[i] * sum_j count_ij col_scale[j] B[j, :] (FP64, row-major,
+ * k columns; NULL scales mean 1): both sparse products of nystrom
+ * (seed^T @ evecs on the landmark CSC, sample @ that on all cells). */
+int snapgpu_csr_spmm_scaled_f64(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_cnt, int64_t n,
+                                const double* d_row_scale, const double* d_col_scale, const double* d_B, int64_t ldb,
+                                int32_t k, double* d_C, int64_t ldc, void* stream);
+/* nystrom's per-chunk normalisation of Q (n x k), chunks of chunk_rows rows:
+ * t = colsum(Q_chunk) * evals; d = Q t; d <= 0 -> the chunk's smallest
+ * positive d; Q /= sqrt(d). SNAPGPU_ERR_DATA if a chunk needs that minimum and
+ * has no positive d. Synchronises. */
+int64_t snapgpu_nystrom_normalize_workspace_bytes(int64_t n, int32_t k, int64_t chunk_rows);
+int snapgpu_nystrom_normalize(double* d_Q, int64_t ldq, int64_t n, int32_t k, const double* d_evals,
+                              int64_t chunk_rows, void* d_ws, int64_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,3 +108,50 @@ def subspace_angle(U: np.ndarray, V: np.ndarray) -> float:
     qv, _ = np.linalg.qr(V)
     s = np.linalg.svd(qu.T @ qv, compute_uv=False)
     return float(np.arccos(np.clip(s.min(), -1.0, 1.0)))
+
+
+def nystrom(X, landmark_rows, n_components: int, chunk_size: int = 20000, feature_weights=None):
+    """spectral_embedding_nystrom (snapatac2-python/src/embedding.rs:60-131) given
+    the landmark rows, then nystrom (:193-268) over row chunks of chunk_size:
+      weights = idf_from_chunks (:288-312) over all rows, or the given weights;
+      seed = normalize(X[landmarks], weights); (v, u, dinv) = spectral_mf(seed);
+      u rows * 1/sqrt(dinv), u columns / v;
+      per chunk: q = sample @ (seed.T @ u); t = q.sum(0) * v; d = q @ t;
+      d[d <= 0] = min(d[d > 0]); q /= sqrt(d).
+    X: cells x selected features, integer counts. Returns (v, q)."""
+    X = sp.csr_matrix(X, dtype=np.float64)
+    X.sort_indices()
+    w = idf(X) if feature_weights is None else np.asarray(feature_weights, dtype=np.float64)
+    seed = normalize(X[np.asarray(landmark_rows)], w)
+    col_sum = np.asarray(seed.sum(axis=0)).ravel()
+    dinv = 1.0 / (seed @ col_sum - 1.0)
+    seed_h = sp.csr_matrix(sp.diags(np.sqrt(dinv)) @ seed)
+    v, u = eigen(seed_h, dinv, n_components, 0)
+    u = u * (1.0 / np.sqrt(dinv)).reshape((-1, 1))
+    u = u / v.reshape((1, -1))
+    out = []
+    for a in range(0, X.shape[0], chunk_size):
+        sample = normalize(X[a:a + chunk_size], w)
+        q = sample @ (seed.T @ u)
+        t = q.sum(axis=0) * v
+        d = q @ t.reshape((-1, 1))
+        d[d <= 0] = np.min(d[d > 0])
+        np.divide(q, np.sqrt(d), out=q)
+        out.append(q)
+    return v, np.vstack(out)
+
+
+def orthogonalize(evals, evecs):
+    """orthogonalize (snapatac2-python/python/snapatac2/tools/_embedding.py:397-413)."""
+    _, sigma, Vt = np.linalg.svd(evecs)
+    V = Vt.T
+    B = np.multiply(V.T, evals.reshape((1, -1))) @ V
+    np.multiply(B, sigma.reshape((-1, 1)), out=B)
+    np.multiply(B, sigma.reshape((1, -1)), out=B)
+    evals_new, evecs_new = np.linalg.eig(B)
+    ix = evals_new.argsort()[::-1]
+    evals_new = evals_new[ix]
+    evecs_new = evecs_new[:, ix]
+    np.divide(evecs_new, sigma.reshape((-1, 1)), out=evecs_new)
+    evecs_new = evecs @ V @ evecs_new
+    return (evals_new, evecs_new)

@@ -77,6 +77,14 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_frag_workspace_bytes": (_i64, [_i64, _i64]),
     "snapgpu_frag_index": (C.c_int, [_p, _i64, _i64, _p, _p, _p, _p, _i64, _p, _p]),
     "snapgpu_frag_fill": (C.c_int, [_p, _i64, _p, _i64, _p, _p, _p, _p, _p, _p, _i32, _p]),
+    "snapgpu_csr_take_rows_workspace_bytes": (_i64, [_i64]),
+    "snapgpu_csr_take_rows_count": (C.c_int, [_p, _i64, _p, _i64, _p, _p, _i64, C.POINTER(_i64), _p]),
+    "snapgpu_csr_take_rows_fill": (C.c_int, [_p, _p, _p, _p, _i64, _p, _p, _p, _p]),
+    "snapgpu_csr_col_counts": (C.c_int, [_p, _i64, _p, _i64, _i64, _p, _p, _p]),
+    "snapgpu_csr_row_inv_norm": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p]),
+    "snapgpu_csr_spmm_scaled_f64": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i64, _i32, _p, _i64, _p]),
+    "snapgpu_nystrom_normalize_workspace_bytes": (_i64, [_i64, _i32, _i64]),
+    "snapgpu_nystrom_normalize": (C.c_int, [_p, _i64, _i64, _i32, _p, _i64, _p, _i64, _p]),
     "snapgpu_frag_format_workspace_bytes": (_i64, [_i64]),
     "snapgpu_frag_format": (C.c_int, [_p, _i64, _p, _p, _i64, _i32, _p, _p, _i32, _p, _p, _p, _i64,
                                       _p, _i64, C.POINTER(_i64), _p]),

@@ -77,10 +77,12 @@ def spectral(
 ) -> tuple[np.ndarray, np.ndarray] | None:
     """Laplacian eigenmaps of the cosine similarity graph (drop-in for snapatac2.tl.spectral).
 
-    Runs on the GPU for distance_metric='cosine' with the full matrix
-    (sample_size=None); the Jaccard metric and the Nystrom approximation are not
-    provided. Eigenvector signs follow a fixed convention (largest-magnitude
-    entry positive), so they may differ from ARPACK's arbitrary signs.
+    Runs on the GPU for distance_metric='cosine', with the full matrix or, for
+    sample_size < n_obs, the Nystrom approximation (landmarks drawn with numpy's
+    PCG64 seeded 2023 where the reference uses Rust's StdRng, so the landmark
+    set differs; see tl/_nystrom.py). The Jaccard metric is not provided.
+    Eigenvector signs follow a fixed convention (largest-magnitude entry
+    positive), so they may differ from ARPACK's arbitrary signs.
     """
     np.random.seed(random_state)
     if isinstance(features, str):
@@ -102,11 +104,21 @@ def spectral(
             raise ValueError("when sample_size is a float, it should be > 0 and <= 1")
         else:
             sample_size = int(sample_size * n_sample)
-    if sample_size < n_sample:
-        raise NotImplementedError("the Nystrom approximation (sample_size < n_obs) is not provided")
     if distance_metric != "cosine":
         raise NotImplementedError("only distance_metric='cosine' is provided")
-    evals, evecs = spectral_embedding(adata, features, n_comps, random_state, feature_weights)
+    if sample_size < n_sample:
+        # _embedding.py:257-265: Nystrom, then orthogonalize
+        from ._nystrom import orthogonalize, spectral_embedding_nystrom
+        torch = _lib.require_gpu()
+        X = _to_device_csr(adata.X, torch)
+        cols, _ = _selected_columns(features, X.shape[1])
+        # the reference's Nystrom call takes no feature weights: always IDF
+        v, u = spectral_embedding_nystrom(select_columns(X, cols), n_comps, sample_size,
+                                          sample_method != "random", chunk_size)
+        evals, evecs = orthogonalize(v, u)
+        evecs = _fix_signs(evecs.cpu().numpy())
+    else:
+        evals, evecs = spectral_embedding(adata, features, n_comps, random_state, feature_weights)
     if weighted_by_sd:
         idx = [i for i in range(evals.shape[0]) if evals[i] > 0]
         evals = evals[idx]

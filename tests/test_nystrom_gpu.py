@@ -1,0 +1,74 @@
+"""Nystrom spectral embedding on the GPU against the oracle's restatement of
+spectral_embedding_nystrom / nystrom (snapatac2-python/src/embedding.rs:60-131,
+193-268) on the same landmark cells, and orthogonalize
+(snapatac2-python/python/snapatac2/tools/_embedding.py:397-413)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import spectral as osp
+
+pytestmark = pytest.mark.gpu
+
+
+def _clustered_counts(n=3000, m=1500, clusters=4, seed=0):
+    rng = np.random.default_rng(seed)
+    prof = rng.gamma(0.3, 1.0, size=(clusters, m))
+    prof /= prof.sum(axis=1, keepdims=True)
+    lab = rng.integers(0, clusters, n)
+    depth = rng.integers(80, 400, n)
+    rows = [rng.multinomial(depth[i], prof[lab[i]]) for i in range(n)]
+    return sp.csr_matrix(np.asarray(rows, dtype=np.int64))
+
+
+def _aligned_diff(A, B):
+    s = np.sign(np.sum(A * B, axis=0))
+    s[s == 0] = 1.0
+    return np.max(np.abs(A * s - B)) / np.max(np.abs(B))
+
+
+@pytest.mark.parametrize("sample,chunk,k", [(800, 20000, 8), (1200, 700, 6)])
+def test_nystrom_matches_oracle(torch_gpu, sample, chunk, k):
+    from snapatac2_amd.tl._nystrom import landmarks, spectral_embedding_nystrom
+    from snapatac2_amd.tl._operator import _to_device_csr
+    X = _clustered_counts()
+    rows = landmarks(X.shape[0], sample)
+    assert rows.size == sample and np.all(np.diff(rows) > 0)
+    v, q = spectral_embedding_nystrom(_to_device_csr(X, torch_gpu), k, sample, chunk_size=chunk,
+                                      landmark_rows=rows)
+    ov, oq = osp.nystrom(X, rows, k, chunk_size=chunk)
+    np.testing.assert_allclose(v, ov, rtol=1e-5, atol=1e-7)
+    q = q.cpu().numpy()
+    assert osp.subspace_angle(q, oq) < 1e-4
+    assert _aligned_diff(q, oq) < 1e-4
+
+
+def test_orthogonalize_matches_oracle(torch_gpu):
+    torch = torch_gpu
+    from snapatac2_amd.tl._nystrom import orthogonalize
+    rng = np.random.default_rng(3)
+    E = rng.standard_normal((5000, 7)) @ rng.standard_normal((7, 7))
+    lam = np.sort(rng.random(7))[::-1]
+    ge, gv = orthogonalize(lam, torch.from_numpy(E).cuda())
+    oe, ov = osp.orthogonalize(lam, E)
+    np.testing.assert_allclose(ge, oe.real, rtol=1e-10, atol=1e-12)
+    gv = gv.cpu().numpy()
+    assert _aligned_diff(gv, ov.real) < 1e-8
+    # orthonormal columns; the same low-rank matrix E diag(lam) E^T
+    np.testing.assert_allclose(gv.T @ gv, np.eye(7), atol=1e-10)
+    np.testing.assert_allclose((gv * ge) @ gv.T, (E * lam) @ E.T, atol=1e-8 * np.abs(E).max() ** 2)
+
+
+@pytest.mark.parametrize("method", ["random", "degree"])
+def test_spectral_nystrom_end_to_end(torch_gpu, method):
+    from snapatac2_amd.anndata import AnnData
+    from snapatac2_amd.tl import spectral
+    X = _clustered_counts(2000, 1000, seed=1)
+    ad = AnnData(X=X)
+    evals, evecs = spectral(ad, n_comps=6, features=None, sample_size=0.4, sample_method=method,
+                            inplace=False)
+    assert evecs.shape[0] == 2000 and evecs.shape[1] == evals.size and evals.size >= 4
+    assert np.all(np.diff(evals) <= 0)
+    # the leading embedding separates the clusters as well as the exact one
+    ex, eu = osp.spectral(X, n_comps=6)
+    assert osp.subspace_angle(evecs[:, :3], eu[:, :3]) < 0.2

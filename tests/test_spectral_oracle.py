@@ -77,3 +77,17 @@ def test_features_mask_and_determinism():
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     c = osp.spectral(X[:, mask], 8)
     np.testing.assert_allclose(a[0], c[0], rtol=1e-12)
+
+
+def test_orthogonalize_oracle_invariants():
+    """orthogonalize (_embedding.py:397-413) returns orthonormal vectors that keep
+    the low-rank matrix E diag(lam) E^T: evecs_new = U_svd W with W orthogonal,
+    and U_svd W diag(lam_new) W^T U_svd^T = U_svd S V^T diag(lam) V S U_svd^T."""
+    rng = np.random.default_rng(5)
+    E = rng.standard_normal((400, 6)) @ rng.standard_normal((6, 6))
+    lam = np.sort(rng.random(6))[::-1]
+    ev, U = osp.orthogonalize(lam, E)
+    ev, U = ev.real, U.real
+    np.testing.assert_allclose(U.T @ U, np.eye(6), atol=1e-10)
+    np.testing.assert_allclose((U * ev) @ U.T, (E * lam) @ E.T, atol=1e-9 * np.abs(E).max() ** 2)
+    assert np.all(np.diff(ev) <= 0)
