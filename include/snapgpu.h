@@ -458,6 +458,43 @@ int64_t snapgpu_nystrom_normalize_workspace_bytes(int64_t n, int32_t k, int64_t 
 int snapgpu_nystrom_normalize(double* d_Q, int64_t ldq, int64_t n, int32_t k, const double* d_evals,
                               int64_t chunk_rows, void* d_ws, int64_t ws_bytes, void* stream);
 
+/* ---- Cell x peak matrix ---------------------------------------------------------
+ * Replaces internal.mk_peak_matrix (snapatac2-python/src/preprocessing.rs:370-420) ->
+ * create_peak_matrix (snapatac2-core/src/feature_count/matrix.rs:95-171) with
+ * RegionCounter (feature_count/counter.rs:73-159) over the fragments of
+ * obsm['fragment_paired'|'fragment_single'] (data_iter.rs:27-123).
+ * The region index (device arrays, built on the host from the peak list):
+ * regions grouped by chromosome in the stored chromosome order, sorted by start
+ * inside a chromosome, pmax = running maximum of end inside the chromosome,
+ * orig = the region's column (first-occurrence order of the distinct regions),
+ * chrom_off[n_chrom + 1] = each chromosome's range, base[n_chrom + 1] = genome
+ * offsets of the chromosomes. Sequence: snapgpu_peak_count (increments per
+ * fragment, scanned; total to *h_total) -> snapgpu_peak_pairs (keys sorted and
+ * grouped; distinct (cell, region) pairs to *h_nnz) -> snapgpu_peak_csr. */
+typedef struct snapgpu_peak_index {
+  const uint64_t* start;
+  const uint64_t* end;
+  const uint64_t* pmax;
+  const uint32_t* orig;
+  const int64_t* chrom_off;
+  const uint64_t* base;
+  int32_t n_chrom;
+} snapgpu_peak_index;
+
+int64_t snapgpu_peak_count_workspace_bytes(int64_t n_fragments);
+int snapgpu_peak_count(const snapgpu_peak_index* index, const uint32_t* d_pos, const uint32_t* d_val,
+                       int64_t n_fragments, int32_t paired, int32_t strategy, uint32_t min_len, uint32_t max_len,
+                       int64_t* d_hits, void* d_ws, int64_t ws_bytes, int64_t* h_total, void* stream);
+int64_t snapgpu_peak_pairs_workspace_bytes(int64_t total);
+int snapgpu_peak_pairs(const snapgpu_peak_index* index, const int64_t* d_cell_ptr, int64_t n_cells,
+                       const uint32_t* d_pos, const uint32_t* d_val, int64_t n_fragments, int32_t paired,
+                       int32_t strategy, uint32_t min_len, uint32_t max_len, const int64_t* d_hits, int64_t total,
+                       uint64_t* d_keys, void* d_ws, int64_t ws_bytes, int64_t* h_nnz, void* stream);
+/* d_indptr[n_cells + 1], d_idx[nnz] (int32 region columns, ascending per row),
+ * d_val[nnz] (uint32 counts), from the workspace of snapgpu_peak_pairs. */
+int snapgpu_peak_csr(const void* d_ws, int64_t ws_bytes, int64_t total, int64_t nnz, int64_t n_cells,
+                     int64_t* d_indptr, int32_t* d_idx, uint32_t* d_val, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,13 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_csr_spmm_scaled_f64": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i64, _i32, _p, _i64, _p]),
     "snapgpu_nystrom_normalize_workspace_bytes": (_i64, [_i64, _i32, _i64]),
     "snapgpu_nystrom_normalize": (C.c_int, [_p, _i64, _i64, _i32, _p, _i64, _p, _i64, _p]),
+    "snapgpu_peak_count_workspace_bytes": (_i64, [_i64]),
+    "snapgpu_peak_count": (C.c_int, [_p, _p, _p, _i64, _i32, _i32, _u32, _u32, _p, _p, _i64,
+                                     C.POINTER(_i64), _p]),
+    "snapgpu_peak_pairs_workspace_bytes": (_i64, [_i64]),
+    "snapgpu_peak_pairs": (C.c_int, [_p, _p, _i64, _p, _p, _i64, _i32, _i32, _u32, _u32, _p, _i64, _p, _p,
+                                     _i64, C.POINTER(_i64), _p]),
+    "snapgpu_peak_csr": (C.c_int, [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p]),
     "snapgpu_frag_format_workspace_bytes": (_i64, [_i64]),
     "snapgpu_frag_format": (C.c_int, [_p, _i64, _p, _p, _i64, _i32, _p, _p, _i32, _p, _p, _p, _i64,
                                       _p, _i64, C.POINTER(_i64), _p]),
@@ -97,6 +104,12 @@ class PanelLayout(C.Structure):
                 ("chunk_panel", _p), ("n_chunks", _i64), ("row_pair_ptr", _p),
                 ("row_pair_ids", _p), ("n_pairs", _i64), ("n_rows", _i64), ("dense_rows", _i64),
                 ("packed", _i32), ("value_type", _i32)]
+
+
+class PeakIndex(C.Structure):
+    """snapgpu_peak_index (include/snapgpu.h): device pointers and the chromosome count."""
+    _fields_ = [("start", _p), ("end", _p), ("pmax", _p), ("orig", _p), ("chrom_off", _p), ("base", _p),
+                ("n_chrom", _i32)]
 
 
 class FragOptions(C.Structure):

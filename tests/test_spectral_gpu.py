@@ -130,7 +130,10 @@ def test_errors(torch_gpu):
     with pytest.raises(ValueError):
         snap.tl.spectral(ad, features=None, sample_size=1)
     with pytest.raises(NotImplementedError):
-        snap.tl.spectral(ad, features=None, sample_size=0.5)
+        snap.tl.spectral(ad, features=None, distance_metric="jaccard")
+    # sample_size below n_obs takes the Nystrom branch
+    ev, U = snap.tl.spectral(ad, features=None, n_comps=5, sample_size=0.5, inplace=False)
+    assert U.shape[0] == 100 and U.shape[1] == ev.size
 
 
 def test_panel_segmented_operator(torch_gpu):
