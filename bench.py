@@ -337,6 +337,56 @@ def nystrom_bench(torch, Xs, n_comps, sample_size, cpu_cells):
     return out
 
 
+def peak_bench(torch, frags, cpu_cells):
+    """Cell x peak matrix (make_peak_matrix, paired-insertion counting) of one
+    resident fragment chunk against 50,000 regions of 500 bp centred on the
+    generator's peak centres (fixed-width peaks, as merged peak sets are).
+    CPU baseline: the C++ restatement (OpenMP over cells) on the first
+    cpu_cells cells, whose rows it must reproduce."""
+    from snapatac2_amd import synth
+    from snapatac2_amd.genome import synthetic_genome
+    from snapatac2_amd.pp._peaks import make_peak_counts
+    cs = synthetic_genome()
+    chroms = list(cs)
+    base = np.concatenate([[0], np.cumsum([cs[c] for c in chroms])])
+    centres = np.sort(synth.GeneratorTables(0).peak_centre.astype(np.int64))
+    k = np.searchsorted(base, centres, side="right") - 1
+    local = centres - base[k]
+    peaks = [f"{chroms[a]}:{max(int(c) - 250, 0)}-{int(c) + 250}" for a, c in zip(k, local)]
+    ts = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        X, names = make_peak_counts(frags, cs, peaks)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    n = frags.shape[0]
+    dt = ts[-1]
+    out = {"workload": f"{n} cells, {frags.n_fragments} fragments, {len(names)} regions of 500 bp, "
+                       f"paired-insertion", "s": round(dt, 4), "cells_per_s": round(n / dt, 1),
+           "fragments_per_s": round(frags.n_fragments / dt, 1), "nnz": X.nnz}
+    log(f"[bench] peak matrix: {dt * 1e3:.1f} ms for {n} cells, nnz {X.nnz}")
+    if cpu_cells:
+        from oracle.peaks import peak_matrix_cpp
+        m = min(cpu_cells, n)
+        cp = frags.cell_ptr[:m + 1].cpu().numpy()
+        nf = int(cp[-1])
+        pos = frags.pos[:nf].cpu().numpy().view(np.uint32)
+        val = frags.values[:nf].cpu().numpy().view(np.uint32)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        ip, ix, d = peak_matrix_cpp(cp, pos, val, True, cs, peaks)
+        cdt = time.perf_counter() - t0
+        ok = (np.array_equal(ip, X.indptr[:m + 1].cpu().numpy())
+              and np.array_equal(ix, X.indices[:int(ip[-1])].cpu().numpy().astype(np.int64))
+              and np.array_equal(d, X.data[:int(ip[-1])].cpu().numpy().view(np.uint32)))
+        out["cpu_baseline"] = {"value": round(m / cdt, 1), "unit": "cells/s", "cores": threads, "kind": "port",
+                               "sample": f"first {m} cells ({nf} fragments)", "s": round(cdt, 2), "agrees": bool(ok)}
+        out["vs_cpu"] = round(out["cells_per_s"] / out["cpu_baseline"]["value"], 1)
+        log(f"[bench] peak matrix CPU baseline: {cdt:.2f}s for {m} cells, agrees {ok}")
+    return out
+
+
 def cpu_baseline(torch, chunks, cfg, emb, sel_nnz_full):
     """Reference algorithm (oracle restatements) timed on this host's cores on a
     bounded sample, extrapolated to the workload's cell count."""
@@ -430,6 +480,8 @@ def main():
                     help="cells of the fragment-import measurement (0: skip it)")
     ap.add_argument("--nystrom-landmarks", type=int, default=100_000,
                     help="landmarks of the Nystrom measurement on the pipeline's matrix (0: skip it)")
+    ap.add_argument("--no-peak-bench", dest="peak_bench", action="store_false",
+                    help="skip the cell x peak matrix measurement")
     ap.add_argument("--import-only", action="store_true",
                     help="run only the fragment-import measurement (its own JSON line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -524,6 +576,9 @@ def main():
         nys = nystrom_bench(torch, nys_X, cfg[2], args.nystrom_landmarks,
                             0 if args.no_cpu_baseline else 20000)
     del nys_X
+    peaks = None
+    if args.peak_bench:
+        peaks = peak_bench(torch, chunks[0][1], 0 if args.no_cpu_baseline else 40000)
     fimp = None
     if args.import_cells:
         fimp = fragment_import(torch, args.import_cells, 0 if args.no_cpu_baseline else 10000)
@@ -555,6 +610,7 @@ def main():
         "knn_precision": kprec,
         "fragment_import": fimp,
         "nystrom": nys,
+        "peak_matrix": peaks,
         "cpu_baseline": cpu,
         "spmm_gbps": roof["achieved"] if roof else None,
         "knn_recall_sample": recall,
