@@ -169,8 +169,13 @@ def make_peak_matrix(
     X, names = make_peak_counts(frags, read_chrom_sizes(adata.uns), list(peaks), counting_strategy,
                                 min_frag_size, max_frag_size)
     if inplace:
-        adata.X = X
-        adata.set_n_vars(len(names), names) if hasattr(adata, "set_n_vars") else None
+        if isinstance(adata, AnnData):
+            adata.set_n_vars(len(names), names)
+            adata.X = X
+        else:  # a real anndata.AnnData
+            import pandas as pd
+            adata.X = X.to_scipy()
+            adata.var_names = pd.Index(names)
         return None
     out = AnnData(X=X, obs=adata.obs.copy() if hasattr(adata, "obs") else None)
     out.set_n_vars(len(names), names)
