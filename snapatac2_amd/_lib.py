@@ -92,6 +92,9 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_peak_pairs": (C.c_int, [_p, _p, _i64, _p, _p, _i64, _i32, _i32, _u32, _u32, _p, _i64, _p, _p,
                                      _i64, C.POINTER(_i64), _p]),
     "snapgpu_peak_csr": (C.c_int, [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p]),
+    "snapgpu_group_max_workspace_bytes": (_i64, [_i64]),
+    "snapgpu_group_max": (C.c_int, [_p, _i64, _p, _p, _i64, _p, _p, _i64, C.POINTER(_i64), _p]),
+    "snapgpu_group_max_csr": (C.c_int, [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p]),
     "snapgpu_frag_format_workspace_bytes": (_i64, [_i64]),
     "snapgpu_frag_format": (C.c_int, [_p, _i64, _p, _p, _i64, _i32, _p, _p, _i32, _p, _p, _p, _i64,
                                       _p, _i64, C.POINTER(_i64), _p]),

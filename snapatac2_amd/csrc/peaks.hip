@@ -176,6 +176,37 @@ __global__ void csr_indptr_kernel(const uint64_t* __restrict__ ukey, int64_t nu,
   indptr[c] = a;
 }
 
+// Gene level of the gene matrix: entry e of a cells x regions CSR becomes the
+// key (row << 32 | group[column]) with its count.
+__global__ void group_key_kernel(const int64_t* __restrict__ indptr, int64_t n_rows, const int32_t* __restrict__ idx,
+                                 const uint32_t* __restrict__ val, int64_t nnz, const uint32_t* __restrict__ group,
+                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  int64_t lo = 0, hi = n_rows;  // last row with indptr[row] <= e
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) / 2;
+    if (indptr[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  keys[e] = ((uint64_t)lo << 32) | group[idx[e]];
+  vals[e] = val[e];
+}
+
+// Largest count of each run of equal keys (GeneCount::get_values keeps the
+// maximum over a gene's transcripts, counter.rs:270-283).
+__global__ void run_max_kernel(const uint64_t* __restrict__ ukey, const int64_t* __restrict__ ustart, int64_t nu,
+                               int64_t total, const uint32_t* __restrict__ vals, int32_t* __restrict__ idx,
+                               uint32_t* __restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nu) return;
+  const int64_t a = ustart[r], b = r + 1 < nu ? ustart[r + 1] : total;
+  uint32_t m = 0;
+  for (int64_t j = a; j < b; ++j) m = vals[j] > m ? vals[j] : m;
+  idx[r] = (int32_t)(ukey[r] & 0xFFFFFFFFull);
+  out[r] = m;
+}
+
 inline int grid(int64_t n) { return (int)ceil_div(n > 0 ? n : 1, kThreads); }
 
 inline Index to_index(const snapgpu_peak_index* p) {
@@ -284,6 +315,84 @@ int snapgpu_peak_csr(const void* d_ws, int64_t ws_bytes, int64_t total, int64_t 
     SG_LAUNCH_CHECK();
   }
   csr_indptr_kernel<<<grid(n_cells + 1), kThreads, 0, st>>>(ukey, nnz, n_cells, d_indptr);
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
+int64_t snapgpu_group_max_workspace_bytes(int64_t nnz) {
+  const int64_t t = nnz > 0 ? nnz : 1;
+  size_t tmp = 0;
+  hipcub::DoubleBuffer<uint64_t> kb(nullptr, nullptr);
+  hipcub::DoubleBuffer<uint32_t> vb(nullptr, nullptr);
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kb, vb, (int)(t < INT32_MAX ? t : INT32_MAX), 0, 64);
+  // unique keys, run starts, their values, keys x2, values x2, run ids
+  return 8 * t * 5 + 4 * t * 3 + snapgpu_scan_workspace_bytes(t) + (int64_t)tmp + 10 * 256;
+}
+
+int snapgpu_group_max(const int64_t* d_indptr, int64_t n_rows, const int32_t* d_idx, const uint32_t* d_val,
+                      int64_t nnz, const uint32_t* d_group, void* d_ws, int64_t ws_bytes, int64_t* h_nnz_out,
+                      void* stream) {
+  clear_error();
+  SG_ARG(n_rows >= 1 && n_rows < (1ll << 31) && nnz >= 0 && nnz < (int64_t)INT32_MAX && h_nnz_out != nullptr,
+         "bad arguments");
+  SG_ARG(ws_bytes >= snapgpu_group_max_workspace_bytes(nnz), "workspace too small");
+  *h_nnz_out = 0;
+  if (nnz == 0) return SNAPGPU_OK;
+  hipStream_t st = as_stream(stream);
+  char* p = (char*)(((uintptr_t)d_ws + 255) & ~(uintptr_t)255);
+  auto take = [&](int64_t bytes) {
+    char* at = p;
+    p += (bytes + 255) & ~(int64_t)255;
+    return at;
+  };
+  uint64_t* ukey = reinterpret_cast<uint64_t*>(take(8 * nnz));
+  int64_t* ustart = reinterpret_cast<int64_t*>(take(8 * nnz));
+  uint32_t* uval = reinterpret_cast<uint32_t*>(take(4 * nnz));
+  uint64_t* kA = reinterpret_cast<uint64_t*>(take(8 * nnz));
+  uint64_t* kB = reinterpret_cast<uint64_t*>(take(8 * nnz));
+  uint32_t* vA = reinterpret_cast<uint32_t*>(take(4 * nnz));
+  uint32_t* vB = reinterpret_cast<uint32_t*>(take(4 * nnz));
+  int64_t* rid = reinterpret_cast<int64_t*>(take(8 * nnz));
+  void* sws = take(snapgpu_scan_workspace_bytes(nnz));
+  void* tmp = p;
+  size_t tmp_bytes = (size_t)(ws_bytes - (p - (char*)d_ws));
+  group_key_kernel<<<grid(nnz), kThreads, 0, st>>>(d_indptr, n_rows, d_idx, d_val, nnz, d_group, kA, vA);
+  SG_LAUNCH_CHECK();
+  int end_bit = 33;
+  while (end_bit < 64 && ((uint64_t)n_rows >> (end_bit - 32)) != 0) ++end_bit;
+  hipcub::DoubleBuffer<uint64_t> kb(kA, kB);
+  hipcub::DoubleBuffer<uint32_t> vb(vA, vB);
+  SG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kb, vb, (int)nnz, 0, end_bit, st));
+  run_flag_kernel<<<grid(nnz), kThreads, 0, st>>>(kb.Current(), nnz, rid);
+  SG_LAUNCH_CHECK();
+  int rc = snapgpu_scan_inclusive_i64(rid, nnz, sws, snapgpu_scan_workspace_bytes(nnz), stream);
+  if (rc) return rc;
+  run_start_kernel<<<grid(nnz), kThreads, 0, st>>>(kb.Current(), nnz, rid, ukey, ustart);
+  SG_LAUNCH_CHECK();
+  // the sorted values are kept next to the run starts for snapgpu_group_max_csr
+  SG_HIP(hipMemcpyAsync(uval, vb.Current(), 4 * nnz, hipMemcpyDeviceToDevice, st));
+  SG_HIP(hipMemcpyAsync(h_nnz_out, rid + nnz - 1, 8, hipMemcpyDeviceToHost, st));
+  SG_HIP(hipStreamSynchronize(st));
+  return SNAPGPU_OK;
+}
+
+int snapgpu_group_max_csr(const void* d_ws, int64_t ws_bytes, int64_t nnz, int64_t nnz_out, int64_t n_rows,
+                          int64_t* d_indptr, int32_t* d_idx, uint32_t* d_val, void* stream) {
+  clear_error();
+  SG_ARG(nnz >= 0 && nnz_out >= 0 && nnz_out <= nnz && n_rows >= 1 && d_indptr != nullptr, "bad arguments");
+  SG_ARG(ws_bytes >= snapgpu_group_max_workspace_bytes(nnz), "workspace too small");
+  hipStream_t st = as_stream(stream);
+  const char* p = (const char*)(((uintptr_t)d_ws + 255) & ~(uintptr_t)255);
+  const int64_t t = nnz > 0 ? nnz : 1;
+  const int64_t s8 = (8 * t + 255) & ~(int64_t)255;
+  const uint64_t* ukey = reinterpret_cast<const uint64_t*>(p);
+  const int64_t* ustart = reinterpret_cast<const int64_t*>(p + s8);
+  const uint32_t* uval = reinterpret_cast<const uint32_t*>(p + 2 * s8);
+  if (nnz_out > 0) {
+    run_max_kernel<<<grid(nnz_out), kThreads, 0, st>>>(ukey, ustart, nnz_out, nnz, uval, d_idx, d_val);
+    SG_LAUNCH_CHECK();
+  }
+  csr_indptr_kernel<<<grid(n_rows + 1), kThreads, 0, st>>>(ukey, nnz_out, n_rows, d_indptr);
   SG_LAUNCH_CHECK();
   return SNAPGPU_OK;
 }
