@@ -387,6 +387,72 @@ def peak_bench(torch, frags, cpu_cells):
     return out
 
 
+def synthetic_transcripts(n_genes=20000, seed=0):
+    """A seeded annotation: n_genes genes placed at the generator's peak centres,
+    1-4 transcripts each (100 bp - 50 kb, either strand)."""
+    from snapatac2_amd import synth
+    from snapatac2_amd.genome import synthetic_genome
+    cs = synthetic_genome()
+    chroms = list(cs)
+    base = np.concatenate([[0], np.cumsum([cs[c] for c in chroms])])
+    rng = np.random.default_rng(seed)
+    centres = rng.choice(synth.GeneratorTables(0).peak_centre.astype(np.int64), n_genes, replace=False)
+    tx = []
+    for g, c in enumerate(centres):
+        k = int(np.searchsorted(base, c, side="right")) - 1
+        loc = int(c) - int(base[k])
+        strand = "+" if rng.random() < 0.5 else "-"
+        for t in range(int(rng.integers(1, 5))):
+            left = max(loc + int(rng.integers(-5000, 5000)), 1)
+            right = min(left + int(rng.integers(100, 50000)), cs[chroms[k]])
+            tx.append(dict(chrom=chroms[k], left=left, right=right, strand=strand,
+                           transcript_id=f"T{g}.{t}", gene_name=f"G{g}", gene_id=f"E{g}"))
+    return tx
+
+
+def gene_bench(torch, frags, cpu_cells):
+    """Gene matrix (make_gene_matrix, gene level, paired-insertion, 2 kb upstream
+    with gene bodies) of one resident fragment chunk over a synthetic
+    20,000-gene annotation. CPU baseline: the oracle with the C++ region
+    counter (OpenMP) on the first cpu_cells cells, whose rows it must reproduce."""
+    from snapatac2_amd.genome import synthetic_genome
+    from snapatac2_amd.pp._genes import gene_counts
+    cs = synthetic_genome()
+    tx = synthetic_transcripts()
+    ts = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        X, names, _ = gene_counts(frags, cs, tx)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    n = frags.shape[0]
+    dt = ts[-1]
+    out = {"workload": f"{n} cells, {frags.n_fragments} fragments, {len(tx)} transcripts of {len(names)} genes",
+           "s": round(dt, 4), "cells_per_s": round(n / dt, 1), "nnz": X.nnz}
+    log(f"[bench] gene matrix: {dt * 1e3:.1f} ms for {n} cells, nnz {X.nnz}")
+    if cpu_cells:
+        from oracle.genes import gene_matrix
+        from oracle.peaks import peak_matrix_cpp
+        m = min(cpu_cells, n)
+        cp = frags.cell_ptr[:m + 1].cpu().numpy()
+        nf = int(cp[-1])
+        pos = frags.pos[:nf].cpu().numpy().view(np.uint32)
+        val = frags.values[:nf].cpu().numpy().view(np.uint32)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        ip, ix, d, _ = gene_matrix(cp, pos, val, True, cs, tx, region_counter=peak_matrix_cpp)
+        cdt = time.perf_counter() - t0
+        ok = (np.array_equal(ip, X.indptr[:m + 1].cpu().numpy())
+              and np.array_equal(d, X.data[:int(ip[-1])].cpu().numpy().view(np.uint32)))
+        out["cpu_baseline"] = {"value": round(m / cdt, 1), "unit": "cells/s", "cores": threads, "kind": "port",
+                               "sample": f"first {m} cells ({nf} fragments); region counts in C++, the gene "
+                                         f"maximum in Python", "s": round(cdt, 2), "agrees": bool(ok)}
+        out["vs_cpu"] = round(out["cells_per_s"] / out["cpu_baseline"]["value"], 1)
+        log(f"[bench] gene matrix CPU baseline: {cdt:.2f}s for {m} cells, agrees {ok}")
+    return out
+
+
 def cpu_baseline(torch, chunks, cfg, emb, sel_nnz_full):
     """Reference algorithm (oracle restatements) timed on this host's cores on a
     bounded sample, extrapolated to the workload's cell count."""
@@ -481,7 +547,7 @@ def main():
     ap.add_argument("--nystrom-landmarks", type=int, default=100_000,
                     help="landmarks of the Nystrom measurement on the pipeline's matrix (0: skip it)")
     ap.add_argument("--no-peak-bench", dest="peak_bench", action="store_false",
-                    help="skip the cell x peak matrix measurement")
+                    help="skip the cell x peak and cell x gene matrix measurements")
     ap.add_argument("--import-only", action="store_true",
                     help="run only the fragment-import measurement (its own JSON line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -576,9 +642,10 @@ def main():
         nys = nystrom_bench(torch, nys_X, cfg[2], args.nystrom_landmarks,
                             0 if args.no_cpu_baseline else 20000)
     del nys_X
-    peaks = None
+    peaks = genes = None
     if args.peak_bench:
         peaks = peak_bench(torch, chunks[0][1], 0 if args.no_cpu_baseline else 40000)
+        genes = gene_bench(torch, chunks[0][1], 0 if args.no_cpu_baseline else 4000)
     fimp = None
     if args.import_cells:
         fimp = fragment_import(torch, args.import_cells, 0 if args.no_cpu_baseline else 10000)
@@ -611,6 +678,7 @@ def main():
         "fragment_import": fimp,
         "nystrom": nys,
         "peak_matrix": peaks,
+        "gene_matrix": genes,
         "cpu_baseline": cpu,
         "spmm_gbps": roof["achieved"] if roof else None,
         "knn_recall_sample": recall,

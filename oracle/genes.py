@@ -29,9 +29,12 @@ def promoters(transcripts, upstream, downstream, include_gene_body):
 
 
 def gene_matrix(cell_ptr, pos, val, paired, chrom_sizes, transcripts, id_type="gene", upstream=2000,
-                downstream=0, include_gene_body=True, strategy="paired-insertion", min_len=None, max_len=None):
+                downstream=0, include_gene_body=True, strategy="paired-insertion", min_len=None, max_len=None,
+                region_counter=peak_matrix):
+    """region_counter: oracle.peaks.peak_matrix, or peak_matrix_cpp (the C++
+    restatement) for the timed CPU baseline."""
     regs = promoters(transcripts, upstream, downstream, include_gene_body)
-    ip, ix, d = peak_matrix(cell_ptr, pos, val, paired, chrom_sizes, regs, strategy, min_len, max_len)
+    ip, ix, d = region_counter(cell_ptr, pos, val, paired, chrom_sizes, regs, strategy, min_len, max_len)
     if id_type == "transcript":
         return ip, ix, d, [t["transcript_id"] for t in transcripts]
     genes = {}
