@@ -1,7 +1,7 @@
 """Drop-in for snapatac2.pp.make_gene_matrix (cell x gene activity), on the GPU.
 
-Reference: snapatac2-python/python/snapatac2/preprocessing/_basic.py:706-805
-(Python API), internal.mk_gene_matrix (snapatac2-python/src/preprocessing.rs:430-490),
+Reference: snapatac2-python/python/snapatac2/preprocessing/_basic.py:707-827
+(Python API), internal.mk_gene_matrix (snapatac2-python/src/preprocessing.rs:432-499),
 read_transcripts_from_gtf / _gff and Promoters::new (snapatac2-core/src/genome.rs:
 64-230), create_gene_matrix (feature_count/matrix.rs:173-273), TranscriptCount /
 GeneCount (feature_count/counter.rs:160-290).
@@ -153,7 +153,7 @@ def make_gene_matrix(
     max_frag_size: int | None = None,
     counting_strategy: str = "paired-insertion",
 ):
-    """Cell by gene activity matrix (snapatac2.pp.make_gene_matrix, _basic.py:706-805).
+    """Cell by gene activity matrix (snapatac2.pp.make_gene_matrix, _basic.py:707-827).
 
     gene_anno: a GTF/GFF path, or an object with an `annotation` path (a
     Genome). `use_x` and backed output (`file`) are not provided here.
