@@ -450,6 +450,10 @@ int snapgpu_csr_row_inv_norm(const int64_t* d_indptr, const int32_t* d_idx, cons
 int snapgpu_csr_spmm_scaled_f64(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_cnt, int64_t n,
                                 const double* d_row_scale, const double* d_col_scale, const double* d_B, int64_t ldb,
                                 int32_t k, double* d_C, int64_t ldc, void* stream);
+/* E = U diag(1/evals) with row i divided by sqrt(dinv[i]): nystrom's scaling of the
+ * landmark eigenvectors (embedding.rs:203-214). Row-major FP64. */
+int snapgpu_nystrom_landmark_scale(const double* d_U, int64_t ldu, int64_t n, int32_t k, const double* d_dinv,
+                                   const double* d_evals, double* d_E, int64_t lde, void* stream);
 /* nystrom's per-chunk normalisation of Q (n x k), chunks of chunk_rows rows:
  * t = colsum(Q_chunk) * evals; d = Q t; d <= 0 -> the chunk's smallest
  * positive d; Q /= sqrt(d). SNAPGPU_ERR_DATA if a chunk needs that minimum and

@@ -83,6 +83,7 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_csr_col_counts": (C.c_int, [_p, _i64, _p, _i64, _i64, _p, _p, _p]),
     "snapgpu_csr_row_inv_norm": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p]),
     "snapgpu_csr_spmm_scaled_f64": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i64, _i32, _p, _i64, _p]),
+    "snapgpu_nystrom_landmark_scale": (C.c_int, [_p, _i64, _i64, _i32, _p, _p, _p, _i64, _p]),
     "snapgpu_nystrom_normalize_workspace_bytes": (_i64, [_i64, _i32, _i64]),
     "snapgpu_nystrom_normalize": (C.c_int, [_p, _i64, _i64, _i32, _p, _i64, _p, _i64, _p]),
     "snapgpu_peak_count_workspace_bytes": (_i64, [_i64]),

@@ -159,6 +159,18 @@ __global__ void scale_rows_kernel(double* __restrict__ Q, int64_t ldq, int64_t n
   for (int c = 0; c < k; ++c) Q[i * ldq + c] *= r;
 }
 
+// E[i][c] = U[i][c] / sqrt(dinv[i]) / evals[c] (nystrom, embedding.rs:203-214:
+// rows times the square root of the degree, columns over the eigenvalue).
+__global__ void landmark_scale_kernel(const double* __restrict__ U, int64_t ldu, int64_t n, int k,
+                                      const double* __restrict__ dinv, const double* __restrict__ evals,
+                                      double* __restrict__ E, int64_t lde) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * k) return;
+  const int64_t i = t / k;
+  const int c = (int)(t % k);
+  E[i * lde + c] = U[i * ldu + c] / sqrt(dinv[i]) / evals[c];
+}
+
 inline int waves_grid(int64_t n) { return (int)ceil_div(n > 0 ? n : 1, kThreads / 64); }
 inline int grid(int64_t n) { return (int)ceil_div(n > 0 ? n : 1, kThreads); }
 
@@ -249,6 +261,16 @@ int snapgpu_csr_spmm_scaled_f64(const int64_t* d_indptr, const int32_t* d_idx, c
   if (n == 0) return SNAPGPU_OK;
   spmm_scaled_kernel<<<waves_grid(n), kThreads, 0, as_stream(stream)>>>(d_indptr, d_idx, d_cnt, n, d_row_scale,
                                                                         d_col_scale, d_B, ldb, k, d_C, ldc);
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
+int snapgpu_nystrom_landmark_scale(const double* d_U, int64_t ldu, int64_t n, int32_t k, const double* d_dinv,
+                                   const double* d_evals, double* d_E, int64_t lde, void* stream) {
+  clear_error();
+  SG_ARG(n >= 0 && k >= 1 && ldu >= k && lde >= k && d_U && d_dinv && d_evals && d_E, "bad arguments");
+  if (n == 0) return SNAPGPU_OK;
+  landmark_scale_kernel<<<grid(n * k), kThreads, 0, as_stream(stream)>>>(d_U, ldu, n, k, d_dinv, d_evals, d_E, lde);
   SG_LAUNCH_CHECK();
   return SNAPGPU_OK;
 }

@@ -119,10 +119,13 @@ def spectral_embedding_nystrom(X: DeviceCSR, n_components: int, sample_size: int
     evals, U, _ = eigsh_lm(op, n_components, seed=0)
     ix = np.argsort(evals, kind="stable")[::-1]
     evals = np.ascontiguousarray(evals[ix])
-    U = U[:, torch.from_numpy(ix.copy()).to(dev)]
+    U = U[:, torch.from_numpy(ix.copy()).to(dev)].contiguous()
+    k = U.shape[1]
+    ev = torch.from_numpy(evals).to(dev)
     # nystrom (embedding.rs:203-214): rows * sqrt(deg) (= 1 / sqrt(dinv)), columns / lambda
-    E = (U / torch.sqrt(op.dinv)[:, None] / torch.from_numpy(evals).to(dev)[None, :]).contiguous()
-    k = E.shape[1]
+    E = torch.empty_like(U)
+    _lib.check(L.snapgpu_nystrom_landmark_scale(U.data_ptr(), k, U.shape[0], k, op.dinv.data_ptr(), ev.data_ptr(),
+                                                E.data_ptr(), k, st))
     # Z = seed~^T E = diag(w) X_seed^T diag(s_seed) E, on the landmark CSC
     Z = torch.empty((m, k), dtype=torch.float64, device=dev)
     _lib.check(L.snapgpu_csr_spmm_scaled_f64(op.colptr.data_ptr(), op.rowidx.data_ptr(), op.cval.data_ptr(),
@@ -136,7 +139,6 @@ def spectral_embedding_nystrom(X: DeviceCSR, n_components: int, sample_size: int
     _lib.check(L.snapgpu_csr_spmm_scaled_f64(X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), n,
                                              s_all.data_ptr(), w.data_ptr(), Z.data_ptr(), k, k,
                                              Q.data_ptr(), k, st))
-    ev = torch.from_numpy(evals).to(dev)
     wsb = int(L.snapgpu_nystrom_normalize_workspace_bytes(n, k, chunk_size))
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
     _lib.check(L.snapgpu_nystrom_normalize(Q.data_ptr(), k, n, k, ev.data_ptr(), int(chunk_size), ws.data_ptr(),
