@@ -96,7 +96,7 @@ struct Counters {
   unsigned int max_bc;          // longest barcode (bytes) over kept records
   unsigned int collision;       // two barcodes with one hash next to each other
   unsigned int any_bad;         // a whitelisted group holds a record count_fragments cannot convert
-  unsigned int pad;
+  unsigned int max_size;        // largest end - start (low 32 bits) over kept records
 };
 
 __host__ __device__ inline uint64_t fnv_step(uint64_t h, uint32_t c) { return (h ^ c) * kFnvPrime; }
@@ -265,7 +265,7 @@ __device__ __forceinline__ uint64_t sat_add(uint64_t a, int64_t d) {
 }
 
 struct LineStats {
-  uint32_t n_rec, n_strand, n_keep, max_bc;
+  uint32_t n_rec, n_strand, n_keep, max_bc, max_size;
   unsigned long long first;  // min (line << 1 | has_strand) over records
 };
 
@@ -394,7 +394,10 @@ __device__ __forceinline__ void parse_line(const uint8_t* __restrict__ text, int
     out.bc[i] = bcw;
     out.bch[i] = bch;
     if (err) atomicMin(&cnt->err, ((unsigned long long)i << 8) | err);
-    if (!err && (fl & F_KEEP)) ++ls.n_keep;
+    if (!err && (fl & F_KEEP)) {
+      ++ls.n_keep;
+      ls.max_size = size > ls.max_size ? size : ls.max_size;
+    }
   }
   if (is_rec) {
     ++ls.n_rec;
@@ -413,31 +416,34 @@ __global__ void __launch_bounds__(kThreads) parse_kernel(const uint8_t* __restri
                                                          const uint8_t* __restrict__ chroms, int64_t shift_left,
                                                          int64_t shift_right, ParseOut out, Counters* cnt,
                                                          int64_t* __restrict__ blk) {
-  __shared__ unsigned long long red[5][kThreads / 64];
+  __shared__ unsigned long long red[6][kThreads / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int64_t b0 = (int64_t)blockIdx.x * kLineBlock;
-  LineStats ls{0, 0, 0, 0, ~0ull};
+  LineStats ls{0, 0, 0, 0, 0, ~0ull};
   for (int r = 0; r < kRounds; ++r) {
     const int64_t i = b0 + (int64_t)r * kThreads + threadIdx.x;
     if (i < n_lines) parse_line(text, n_bytes, line_start, i, chroms, shift_left, shift_right, out, cnt, ls);
   }
-  unsigned long long v[5] = {ls.n_rec, ls.n_strand, ls.n_keep, ls.max_bc, ls.first};
+  unsigned long long v[6] = {ls.n_rec, ls.n_strand, ls.n_keep, ls.max_bc, ls.first, ls.max_size};
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) v[q] += __shfl_xor(v[q], d, 64);
-    const unsigned long long m = __shfl_xor(v[3], d, 64), f = __shfl_xor(v[4], d, 64);
+    const unsigned long long m = __shfl_xor(v[3], d, 64), f = __shfl_xor(v[4], d, 64),
+                             z = __shfl_xor(v[5], d, 64);
     v[3] = m > v[3] ? m : v[3];
     v[4] = f < v[4] ? f : v[4];
+    v[5] = z > v[5] ? z : v[5];
   }
   if (lane == 0)
-    for (int q = 0; q < 5; ++q) red[q][wid] = v[q];
+    for (int q = 0; q < 6; ++q) red[q][wid] = v[q];
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < kThreads / 64; ++w) {
       for (int q = 0; q < 3; ++q) v[q] += red[q][w];
       v[3] = red[3][w] > v[3] ? red[3][w] : v[3];
       v[4] = red[4][w] < v[4] ? red[4][w] : v[4];
+      v[5] = red[5][w] > v[5] ? red[5][w] : v[5];
     }
     blk[blockIdx.x] = (int64_t)v[2];
     if (v[0]) atomicAdd(&cnt->n_rec, v[0]);
@@ -446,6 +452,8 @@ __global__ void __launch_bounds__(kThreads) parse_kernel(const uint8_t* __restri
       atomicMax(&cnt->max_bc, (unsigned int)v[3]);
     if (v[4] != ~0ull && v[4] < *reinterpret_cast<volatile unsigned long long*>(&cnt->first))
       atomicMin(&cnt->first, v[4]);
+    if (v[5] && v[5] > *reinterpret_cast<volatile unsigned int*>(&cnt->max_size))
+      atomicMax(&cnt->max_size, (unsigned int)v[5]);
   }
 }
 
@@ -516,22 +524,24 @@ __global__ void iota_kernel(uint32_t* __restrict__ v, int64_t n) {
   if (j < n) v[j] = (uint32_t)j;
 }
 
-// (position, value) key of the record at sorted slot j; records outside the
-// genome sort last inside their group.
+// (position, value) key of the record at sorted slot j, in pos_bits + vbits
+// bits: position above the value, the value offset by max_size for
+// single-end records so that -size .. +size order as integers. Records outside
+// the genome get `outside` (above every genome key) and sort last inside their
+// group.
 __global__ void pv_key_kernel(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ kept,
                               const uint8_t* __restrict__ flags, const uint32_t* __restrict__ pos,
-                              const uint32_t* __restrict__ size, int64_t n, int paired,
-                              uint64_t* __restrict__ key) {
+                              const uint32_t* __restrict__ size, int64_t n, int paired, int vbits,
+                              uint32_t max_size, uint64_t outside, uint64_t* __restrict__ key) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t line = kept[perm[j]];
   const uint8_t f = flags[line];
-  uint64_t k = ~0ull;
+  uint64_t k = outside;
   if (f & F_GENOME) {
-    const uint32_t sz = size[line];
-    const uint32_t signed_val = (f & F_REV) ? (uint32_t)(0u - sz) : sz;
-    const uint32_t v = paired ? sz : (signed_val ^ 0x80000000u);  // int32 order as uint32
-    k = ((uint64_t)pos[line] << 32) | v;
+    const uint64_t sz = size[line];
+    const uint64_t v = paired ? sz : ((f & F_REV) ? (uint64_t)max_size - sz : (uint64_t)max_size + sz);
+    k = ((uint64_t)pos[line] << vbits) | v;
   }
   key[j] = k;
 }
@@ -1244,13 +1254,28 @@ int snapgpu_frag_index(const uint8_t* d_text, int64_t n_bytes, int64_t n_lines, 
   uint64_t* keyB = at<uint64_t>(w, Lo.keyB);
   void* tmp = at<char>(w, Lo.tmp);
   size_t tmp_bytes = Lo.tmp_bytes;
+  // key widths: positions below the genome length, values up to the largest size
+  TableHeader th{};
+  SG_HIP(hipMemcpyAsync(&th, d_chroms, sizeof(TableHeader), hipMemcpyDeviceToHost, st));
+  SG_HIP(hipStreamSynchronize(st));
+  auto bits_of = [](uint64_t x) {
+    int b = 1;
+    while (b < 64 && (x >> b) != 0) ++b;
+    return b;
+  };
+  const uint32_t max_size = ch.max_size;
+  const int vbits = bits_of(paired ? (uint64_t)max_size : 2ull * max_size);
+  const int pos_bits = bits_of(th.total);
+  SG_ARG(pos_bits + vbits <= 64, "position and value keys exceed 64 bits");
+  const uint64_t outside = (uint64_t)th.total << vbits;  // above every genome key (pos < total)
   iota_kernel<<<grid(K), kThreads, 0, st>>>(permA, K);
   SG_LAUNCH_CHECK();
-  pv_key_kernel<<<grid(K), kThreads, 0, st>>>(permA, kept, po.flags, po.pos, po.size, K, paired, keyA);
+  pv_key_kernel<<<grid(K), kThreads, 0, st>>>(permA, kept, po.flags, po.pos, po.size, K, paired, vbits, max_size,
+                                              outside, keyA);
   SG_LAUNCH_CHECK();
   hipcub::DoubleBuffer<uint64_t> kb(keyA, keyB);
   hipcub::DoubleBuffer<uint32_t> vb(permA, permB);
-  SG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kb, vb, (int)K, 0, 64, st));
+  SG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kb, vb, (int)K, 0, pos_bits + vbits, st));
   int64_t* run = at<int64_t>(w, Lo.run);
   int end_bit = 64;
   if (opt->sorted_by_barcode) {
