@@ -139,12 +139,94 @@ def _to_anndata(cell_ptr, pos, val, qc, names, summ, cs, torch):
                    uns={"reference_sequences": reference_sequences(cs)})
 
 
-def _read_text(path) -> bytes:
+TEXT_BLOCK_BYTES = 1 << 31  # text per device call when streaming a barcode-sorted file
+
+
+def _open(path):
     with open(path, "rb") as fh:
         head = fh.read(2)
-    opener = gzip.open if head == b"\x1f\x8b" else open
-    with opener(path, "rb") as fh:
+    return (gzip.open if head == b"\x1f\x8b" else open)(path, "rb")
+
+
+def _read_text(path) -> bytes:
+    with _open(path) as fh:
         return fh.read()
+
+
+def _barcode(line: bytes):
+    """The barcode field of a record line, None for comment / empty / short lines."""
+    if not line or line.startswith(b"#"):
+        return None
+    f = line.rstrip(b"\r").split(b"\t", 4)
+    return f[3] if len(f) > 3 else None
+
+
+def _last_run_start(buf: bytes) -> int:
+    """Offset of the first line of the last barcode run among the complete lines
+    of buf (0 if that run starts at the beginning, -1 without a complete record)."""
+    end = buf.rfind(b"\n")
+    if end < 0:
+        return -1
+    pos, last, start = end, None, -1
+    while pos > 0:
+        prev = buf.rfind(b"\n", 0, pos)
+        line = buf[prev + 1:pos]
+        bc = _barcode(line)
+        if bc is not None:
+            if last is None:
+                last = bc
+            elif bc != last:
+                return start
+            start = prev + 1
+        pos = prev if prev >= 0 else 0
+        if prev < 0:
+            break
+    return 0 if last is not None else -1
+
+
+def _text_blocks(path, block_bytes: int):
+    """Blocks of a barcode-sorted fragment file, each ending where a barcode run
+    ends, so no barcode group is split between two device calls."""
+    carry = b""
+    with _open(path) as fh:
+        while True:
+            data = fh.read(block_bytes)
+            buf = carry + data
+            if not data:
+                if buf:
+                    yield buf
+                return
+            cut = _last_run_start(buf)
+            if cut <= 0:  # one barcode run fills the buffer: read on
+                carry = buf
+                continue
+            yield buf[:cut]
+            carry = buf[cut:]
+
+
+def _concat(parts):
+    """Rows of several imports, in order (fragment stores on the device, obs on the host)."""
+    import pandas as pd
+    torch = _lib.require_gpu()
+    parts = [p for p in parts if p.n_obs > 0]
+    if not parts:
+        log.warning("No barcodes passed the QC filter. No data is imported.")
+        return AnnData(n_obs=0, n_vars=0)
+    if len(parts) == 1:
+        return parts[0]
+    keys = {next(iter(p.obsm)) for p in parts}
+    if len(keys) != 1:
+        raise _lib.PanicException("paired-end and single-end records in one file")
+    key = keys.pop()
+    frs = [p.obsm[key] for p in parts]
+    offs = np.cumsum([0] + [f.n_fragments for f in frs])
+    cell_ptr = torch.cat([frs[0].cell_ptr[:1]] + [f.cell_ptr[1:] + int(o) for f, o in zip(frs, offs[:-1])])
+    frags = DeviceFragments(cell_ptr, torch.cat([f.pos for f in frs]), torch.cat([f.values for f in frs]),
+                            frs[0].n_cols, frs[0].paired)
+    obs = pd.concat([p.obs for p in parts])
+    if obs.index.has_duplicates:  # import.rs:139-141
+        raise _lib.PanicException("Please sort fragment file by barcodes")
+    return AnnData(obs=obs, n_obs=len(obs), n_vars=0, obsm={key: frags}, uns=dict(parts[0].uns))
 
 
 def import_fragments(
@@ -162,6 +244,7 @@ def import_fragments(
     tempdir=None,
     backend="hdf5",
     n_jobs: int = 8,
+    _block_bytes: int | None = None,
 ):
     """Import fragment files and compute basic QC metrics
     (snapatac2.pp.import_fragments, _basic.py:156-340).
@@ -170,7 +253,10 @@ def import_fragments(
     obs 'n_fragment', 'frac_dup', 'frac_mito', uns 'reference_sequences'.
     `chunk_size`, `tempdir` and `n_jobs` only shape the reference's CPU
     processing and do not change the result; backed output (`file`) needs an
-    HDF5 library, which this build does not have.
+    HDF5 library, which this build does not have. A barcode-sorted file is
+    streamed through the device in blocks of about TEXT_BLOCK_BYTES ending at
+    barcode-run boundaries; an unsorted file is imported in one device call
+    (at most 2^31 - 2 lines).
     """
     chrom_sizes = chrom_sizes.chrom_sizes if hasattr(chrom_sizes, "chrom_sizes") else chrom_sizes
     if len(chrom_sizes) == 0:
@@ -191,10 +277,13 @@ def import_fragments(
                 for i, f in enumerate(fragment_file)]
     if file is not None:
         raise NotImplementedError("backed AnnData output needs HDF5, which is not available here")
-    return import_fragment_text(_read_text(fragment_file), chrom_sizes,
-                                min_num_fragments=min_num_fragments,
-                                sorted_by_barcode=sorted_by_barcode, whitelist=whitelist, chrM=chrM,
-                                shift_left=shift_left, shift_right=shift_right)
+    kw = dict(min_num_fragments=min_num_fragments, sorted_by_barcode=sorted_by_barcode, whitelist=whitelist,
+              chrM=chrM, shift_left=shift_left, shift_right=shift_right)
+    if not sorted_by_barcode:  # one device call over the whole text
+        return import_fragment_text(_read_text(fragment_file), chrom_sizes, **kw)
+    # barcode-sorted input streams through the device in blocks of whole barcode runs
+    return _concat([import_fragment_text(b, chrom_sizes, **kw)
+                    for b in _text_blocks(fragment_file, _block_bytes or TEXT_BLOCK_BYTES)])
 
 
 def export_fragment_text(frags: DeviceFragments, chrom_sizes: dict[str, int], barcodes,

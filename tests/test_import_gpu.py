@@ -175,3 +175,31 @@ def test_exclude_at_import_equals_exclude_at_tiling(torch_gpu):
     assert np.array_equal(X1.indices, X2.indices) and np.array_equal(X1.indptr, X2.indptr)
     assert list(a1.obs_names) == list(a2.obs_names)
     assert len(n1) == len(n2) and [n1[i] for i in (0, len(n1) - 1)] == [n2[i] for i in (0, len(n2) - 1)]
+
+
+@pytest.mark.parametrize("gz", [False, True])
+def test_streamed_blocks_equal_one_call(torch_gpu, tmp_path, gz):
+    """import_fragments streams a barcode-sorted file in blocks that end at
+    barcode-run boundaries; with blocks of a few KB the result equals one call
+    over the whole text and the oracle."""
+    import gzip as gzip_mod
+    from snapatac2_amd.pp import import_fragments
+    text = _random_text(11, 20_000, n_bc=300, sorted_bc=True)
+    path = tmp_path / ("f.tsv.gz" if gz else "f.tsv")
+    path.write_bytes(gzip_mod.compress(text) if gz else text)
+    one = import_fragments(path, GENOME, min_num_fragments=3)
+    many = import_fragments(path, GENOME, min_num_fragments=3, _block_bytes=4096)
+    ref = import_text(text, GENOME, min_num_fragments=3)
+    _check(one, ref)
+    _check(many, ref)
+
+
+def test_streamed_duplicate_barcode_is_reported(torch_gpu, tmp_path):
+    from snapatac2_amd._lib import PanicException
+    from snapatac2_amd.pp import import_fragments
+    lines = [f"chr1\t{i}\t{i + 50}\tA" for i in range(200)] + [f"chr1\t{i}\t{i + 50}\tB" for i in range(200)] \
+        + [f"chr1\t{i}\t{i + 50}\tA" for i in range(200)]
+    path = tmp_path / "dup.tsv"
+    path.write_text("\n".join(lines) + "\n")
+    with pytest.raises(PanicException):
+        import_fragments(path, GENOME, min_num_fragments=0, _block_bytes=2048)
