@@ -191,10 +191,13 @@ def eigsh_lm(op, k: int, block: int = 16, max_basis: int | None = None, tol: flo
     steps = 0
     last_res = np.inf
     since_check = 0
+    prefetched = False  # the current block's application was launched before a check
     while True:
         steps += 1
         since_check += 1
-        op.apply(V[:, J - b:J], W, b)
+        if not prefetched:
+            op.apply(V[:, J - b:J], W, b)
+        prefetched = False
         # classical Gram-Schmidt against the basis; one pass leaves components of
         # about eps * |W| / |W1| along it, so a second pass runs only when some
         # column lost more than 95% of its norm in the first
@@ -215,6 +218,11 @@ def eigsh_lm(op, k: int, block: int = 16, max_basis: int | None = None, tol: flo
             J += b
             continue
         since_check = 0
+        if not at_restart:
+            # the next block's application runs on the device while the host solves
+            # the projected problem; a converged check discards it
+            op.apply(V[:, J:J + b], W, b)
+            prefetched = True
         Hs = (H[:J, :J] + H[:J, :J].T) / 2
         theta, S = _eigh_small(Hs)
         order = _lm_order(theta, k)
