@@ -218,11 +218,11 @@ def eigsh_lm(op, k: int, block: int = 16, max_basis: int | None = None, tol: flo
             J += b
             continue
         since_check = 0
-        if not at_restart:
-            # the next block's application runs on the device while the host solves
-            # the projected problem; a converged check discards it
-            op.apply(V[:, J:J + b], W, b)
-            prefetched = True
+        # the next block's application runs on the device while the host solves the
+        # projected problem (a thick restart moves that block to V[:, p:p+b] without
+        # changing it); a converged check discards it
+        op.apply(V[:, J:J + b], W, b)
+        prefetched = True
         Hs = (H[:J, :J] + H[:J, :J].T) / 2
         theta, S = _eigh_small(Hs)
         order = _lm_order(theta, k)
