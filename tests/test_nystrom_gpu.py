@@ -27,11 +27,12 @@ def _aligned_diff(A, B):
     return np.max(np.abs(A * s - B)) / np.max(np.abs(B))
 
 
-@pytest.mark.parametrize("sample,chunk,k", [(800, 20000, 8), (1200, 700, 6)])
-def test_nystrom_matches_oracle(torch_gpu, sample, chunk, k):
+@pytest.mark.parametrize("n,sample,chunk,k", [(3000, 800, 20000, 8), (3000, 1200, 700, 6),
+                                               (20000, 2000, 5000, 10)])
+def test_nystrom_matches_oracle(torch_gpu, n, sample, chunk, k):
     from snapatac2_amd.tl._nystrom import landmarks, spectral_embedding_nystrom
     from snapatac2_amd.tl._operator import _to_device_csr
-    X = _clustered_counts()
+    X = _clustered_counts(n, 1500, 6 if n > 3000 else 4, seed=n)
     rows = landmarks(X.shape[0], sample)
     assert rows.size == sample and np.all(np.diff(rows) > 0)
     v, q = spectral_embedding_nystrom(_to_device_csr(X, torch_gpu), k, sample, chunk_size=chunk,
