@@ -100,3 +100,28 @@ def test_make_peak_matrix_api(torch_gpu):
     assert np.array_equal(out.X.indptr.cpu().numpy(), ip)
     with pytest.raises(RuntimeError):
         make_peak_matrix(ad, use_rep="peaks", peak_file="x.bed")
+
+
+@pytest.mark.parametrize("strategy", ["paired-insertion", "fragment"])
+def test_many_cells_against_cpp(torch_gpu, strategy):
+    """2,000 synthetic cells x 50,000 fixed-width regions at the generator's peak
+    centres, against the C++ restatement (another region search)."""
+    from oracle.peaks import peak_matrix_cpp
+    from snapatac2_amd import synth
+    from snapatac2_amd.genome import synthetic_genome
+    from snapatac2_amd.pp._peaks import make_peak_counts
+    cs = synthetic_genome()
+    chroms = list(cs)
+    base = np.concatenate([[0], np.cumsum([cs[c] for c in chroms])])
+    tab = synth.GeneratorTables(0)
+    centres = np.sort(tab.peak_centre.astype(np.int64))
+    k = np.searchsorted(base, centres, side="right") - 1
+    peaks = [f"{chroms[a]}:{max(int(c - base[a]) - 250, 0)}-{int(c - base[a]) + 250}" for a, c in zip(k, centres)]
+    fr = synth.generate_chunk(tab, 0, 2000)
+    X, _ = make_peak_counts(fr, cs, peaks, strategy)
+    h = fr.to_scipy()
+    ip, ix, d = peak_matrix_cpp(h.indptr, h.indices, h.data, True, cs, peaks, strategy)
+    assert np.array_equal(X.indptr.cpu().numpy(), ip)
+    assert np.array_equal(X.indices.cpu().numpy().astype(np.int64), ix)
+    assert np.array_equal(X.data.cpu().numpy().view(np.uint32), d)
+    assert X.nnz > 100_000
