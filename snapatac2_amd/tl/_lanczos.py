@@ -189,7 +189,7 @@ def eigsh_lm(op, k: int, block: int = 16, max_basis: int | None = None, tol: flo
     J = b
     restarts = 0
     steps = 0
-    last_res = np.inf
+    last_res = prev_res = np.inf
     since_check = 0
     prefetched = False  # the current block's application was launched before a check
     while True:
@@ -220,15 +220,18 @@ def eigsh_lm(op, k: int, block: int = 16, max_basis: int | None = None, tol: flo
         since_check = 0
         # the next block's application runs on the device while the host solves the
         # projected problem (a thick restart moves that block to V[:, p:p+b] without
-        # changing it); a converged check discards it
-        op.apply(V[:, J:J + b], W, b)
-        prefetched = True
+        # changing it); a converged check would discard it, so it is skipped when the
+        # residual's decay over the last two checks predicts convergence here
+        predicted = last_res * (last_res / prev_res) if 0.0 < last_res < prev_res else np.inf
+        if predicted > tol:
+            op.apply(V[:, J:J + b], W, b)
+            prefetched = True
         Hs = (H[:J, :J] + H[:J, :J].T) / 2
         theta, S = _eigh_small(Hs)
         order = _lm_order(theta, k)
         res = np.linalg.norm(R @ S[J - b:J, order], axis=0)
         scale = max(np.abs(theta).max(), 1e-300)
-        last_res = res.max() / scale
+        prev_res, last_res = last_res, res.max() / scale
         if np.all(res <= tol * scale):
             break
         if not at_restart:
