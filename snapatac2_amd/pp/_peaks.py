@@ -52,9 +52,14 @@ class PeakIndex:
         dev = torch.device("cuda")
         chroms = list(chrom_sizes)
         cid = {c: i for i, c in enumerate(chroms)}
-        keep = [(cid[c], s, e, i) for i, (c, s, e) in enumerate(regs) if c in cid]
-        keep.sort(key=lambda t: (t[0], t[1], t[3]))
-        arr = np.asarray(keep, dtype=np.int64).reshape(-1, 4)
+        rc = np.fromiter((cid.get(r[0], -1) for r in regs), dtype=np.int64, count=len(regs))
+        rs = np.fromiter((r[1] for r in regs), dtype=np.int64, count=len(regs))
+        re_ = np.fromiter((r[2] for r in regs), dtype=np.int64, count=len(regs))
+        ri = np.arange(len(regs), dtype=np.int64)
+        on = rc >= 0
+        rc, rs, re_, ri = rc[on], rs[on], re_[on], ri[on]
+        order = np.lexsort((ri, rs, rc))
+        arr = np.stack([rc[order], rs[order], re_[order], ri[order]], axis=1).reshape(-1, 4)
         starts = arr[:, 1].astype(np.uint64)
         ends = arr[:, 2].astype(np.uint64)
         pmax = np.zeros_like(ends)
