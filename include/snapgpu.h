@@ -472,7 +472,8 @@ int snapgpu_nystrom_normalize(double* d_Q, int64_t ldq, int64_t n, int32_t k, co
  * inside a chromosome, pmax = running maximum of end inside the chromosome,
  * orig = the region's column (first-occurrence order of the distinct regions),
  * chrom_off[n_chrom + 1] = each chromosome's range, base[n_chrom + 1] = genome
- * offsets of the chromosomes. Sequence: snapgpu_peak_count (increments per
+ * offsets of the chromosomes, n_columns = the number of region columns (every
+ * orig < n_columns; 0 when unknown, which sorts 32 region bits). Sequence: snapgpu_peak_count (increments per
  * fragment, scanned; total to *h_total) -> snapgpu_peak_pairs (keys sorted and
  * grouped; distinct (cell, region) pairs to *h_nnz) -> snapgpu_peak_csr. */
 typedef struct snapgpu_peak_index {
@@ -483,6 +484,7 @@ typedef struct snapgpu_peak_index {
   const int64_t* chrom_off;
   const uint64_t* base;
   int32_t n_chrom;
+  int32_t n_columns;
 } snapgpu_peak_index;
 
 int64_t snapgpu_peak_count_workspace_bytes(int64_t n_fragments);

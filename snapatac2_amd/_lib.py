@@ -111,9 +111,9 @@ class PanelLayout(C.Structure):
 
 
 class PeakIndex(C.Structure):
-    """snapgpu_peak_index (include/snapgpu.h): device pointers and the chromosome count."""
+    """snapgpu_peak_index (include/snapgpu.h): device pointers, the chromosome and column counts."""
     _fields_ = [("start", _p), ("end", _p), ("pmax", _p), ("orig", _p), ("chrom_off", _p), ("base", _p),
-                ("n_chrom", _i32)]
+                ("n_chrom", _i32), ("n_columns", _i32)]
 
 
 class FragOptions(C.Structure):

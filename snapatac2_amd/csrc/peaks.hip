@@ -13,8 +13,9 @@
 // overlap: regions of a chromosome are sorted by start with a running maximum
 // of their ends, so a query walks down from the last region starting before
 // its end while that maximum passes its start. A count pass sizes the output,
-// an emit pass writes one (cell << 32 | region) key per increment, a radix
-// sort groups them, and the runs of equal keys are the CSR entries and their
+// an emit pass writes one (cell << rbits | region) key per increment (rbits =
+// the bits of the largest column), a radix sort over the rbits + cell bits
+// groups them, and the runs of equal keys are the CSR entries and their
 // counts (columns ascend inside a row, as the reference's BTreeMap gives).
 #include <hipcub/hipcub.hpp>
 
@@ -117,7 +118,7 @@ __global__ void count_kernel(Index I, const uint32_t* __restrict__ pos, const ui
 __global__ void emit_kernel(Index I, const int64_t* __restrict__ cell_ptr, int64_t n_cells,
                             const uint32_t* __restrict__ pos, const uint32_t* __restrict__ val, int64_t n, int paired,
                             int strategy, uint32_t min_len, uint32_t max_len, const int64_t* __restrict__ hits_incl,
-                            int64_t total, uint64_t* __restrict__ keys) {
+                            int64_t total, int rbits, uint64_t* __restrict__ keys) {
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= n) return;
   const Frag q = frag_at(I, pos, val, f, paired, min_len, max_len);
@@ -130,26 +131,29 @@ __global__ void emit_kernel(Index I, const int64_t* __restrict__ cell_ptr, int64
   }
   int64_t o = f == 0 ? 0 : hits_incl[f - 1];
   const int64_t o_end = hits_incl[f];
-  const uint64_t cell = (uint64_t)lo << 32;
+  const uint64_t cell = (uint64_t)lo << rbits;
   visit(I, q, paired, strategy, [&](uint32_t r) {
     if (o < o_end && o < total) keys[o] = cell | r;
     ++o;
   });
 }
 
-// run starts of the sorted keys: flag[j] = 1 where a new key starts
+// run starts of the sorted keys: flag[j] = 1 where a new key starts (the
+// flags depend only on equality, so on neither key layout)
 __global__ void run_flag_kernel(const uint64_t* __restrict__ keys, int64_t n, int64_t* __restrict__ flag) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   flag[j] = (j == 0 || keys[j] != keys[j - 1]) ? 1 : 0;
 }
 
+// distinct keys, widened from (cell << rbits | region) to (cell << 32 | region)
 __global__ void run_start_kernel(const uint64_t* __restrict__ keys, int64_t n, const int64_t* __restrict__ rid,
-                                 uint64_t* __restrict__ ukey, int64_t* __restrict__ ustart) {
+                                 int rbits, uint64_t* __restrict__ ukey, int64_t* __restrict__ ustart) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   if (j == 0 || rid[j] != rid[j - 1]) {
-    ukey[rid[j] - 1] = keys[j];
+    const uint64_t k = keys[j];
+    ukey[rid[j] - 1] = ((k >> rbits) << 32) | (k & ((1ull << rbits) - 1));
     ustart[rid[j] - 1] = j;
   }
 }
@@ -264,6 +268,7 @@ int snapgpu_peak_pairs(const snapgpu_peak_index* index, const int64_t* d_cell_pt
   SG_ARG(index != nullptr && h_nnz != nullptr && total >= 0 && n_cells >= 1 && n_cells < (1ll << 31), "bad arguments");
   SG_ARG(total < (int64_t)INT32_MAX, "at most 2^31 - 1 increments per call");
   SG_ARG(ws_bytes >= snapgpu_peak_pairs_workspace_bytes(total), "workspace too small");
+  SG_ARG(index->n_columns >= 0, "negative column count");
   *h_nnz = 0;
   if (total == 0) return SNAPGPU_OK;
   hipStream_t st = as_stream(stream);
@@ -279,12 +284,18 @@ int snapgpu_peak_pairs(const snapgpu_peak_index* index, const int64_t* d_cell_pt
   void* sws = take(snapgpu_scan_workspace_bytes(total));
   void* tmp = p;
   size_t tmp_bytes = (size_t)(ws_bytes - (p - (char*)d_ws));
+  // every column is below 2^rbits (32 when the caller gives no column count)
+  int rbits = 32;
+  if (index->n_columns > 0) {
+    rbits = 1;
+    while (rbits < 32 && ((uint64_t)(index->n_columns - 1) >> rbits) != 0) ++rbits;
+  }
   emit_kernel<<<grid(n_fragments), kThreads, 0, st>>>(to_index(index), d_cell_ptr, n_cells, d_pos, d_val,
                                                       n_fragments, paired, strategy, min_len, max_len, d_hits, total,
-                                                      d_keys);
+                                                      rbits, d_keys);
   SG_LAUNCH_CHECK();
-  int end_bit = 33;
-  while (end_bit < 64 && ((uint64_t)n_cells >> (end_bit - 32)) != 0) ++end_bit;
+  int end_bit = rbits + 1;  // rbits region bits, then the bits of n_cells
+  while (end_bit < 64 && ((uint64_t)n_cells >> (end_bit - rbits)) != 0) ++end_bit;
   hipcub::DoubleBuffer<uint64_t> kb(d_keys, alt);
   SG_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, kb, (int)total, 0, end_bit, st));
   if (kb.Current() != d_keys) SG_HIP(hipMemcpyAsync(d_keys, kb.Current(), 8 * total, hipMemcpyDeviceToDevice, st));
@@ -293,7 +304,7 @@ int snapgpu_peak_pairs(const snapgpu_peak_index* index, const int64_t* d_cell_pt
   int rc = snapgpu_scan_inclusive_i64(rid, total, sws, snapgpu_scan_workspace_bytes(total), stream);
   if (rc) return rc;
   // unique keys overwrite the alternate buffer; their run starts go to ustart
-  run_start_kernel<<<grid(total), kThreads, 0, st>>>(d_keys, total, rid, alt, ustart);
+  run_start_kernel<<<grid(total), kThreads, 0, st>>>(d_keys, total, rid, rbits, alt, ustart);
   SG_LAUNCH_CHECK();
   SG_HIP(hipMemcpyAsync(h_nnz, rid + total - 1, 8, hipMemcpyDeviceToHost, st));
   SG_HIP(hipStreamSynchronize(st));
@@ -367,7 +378,7 @@ int snapgpu_group_max(const int64_t* d_indptr, int64_t n_rows, const int32_t* d_
   SG_LAUNCH_CHECK();
   int rc = snapgpu_scan_inclusive_i64(rid, nnz, sws, snapgpu_scan_workspace_bytes(nnz), stream);
   if (rc) return rc;
-  run_start_kernel<<<grid(nnz), kThreads, 0, st>>>(kb.Current(), nnz, rid, ukey, ustart);
+  run_start_kernel<<<grid(nnz), kThreads, 0, st>>>(kb.Current(), nnz, rid, 32, ukey, ustart);
   SG_LAUNCH_CHECK();
   // the sorted values are kept next to the run starts for snapgpu_group_max_csr
   SG_HIP(hipMemcpyAsync(uval, vb.Current(), 4 * nnz, hipMemcpyDeviceToDevice, st));

@@ -75,7 +75,7 @@ class PeakIndex:
                       t(pmax if pmax.size else np.zeros(1, np.uint64)),
                       t(arr[:, 3].astype(np.uint32) if arr.size else np.zeros(1, np.uint32)),
                       t(off), t(base)]
-        self.c = _lib.PeakIndex(*[x.data_ptr() for x in self._keep], len(chroms))
+        self.c = _lib.PeakIndex(*[x.data_ptr() for x in self._keep], len(chroms), len(regs))
 
 
 def make_peak_counts(frags, chrom_sizes: dict[str, int], peaks, counting_strategy: str = "paired-insertion",
