@@ -25,7 +25,7 @@ from .._device import DeviceCSR
 from ..anndata import AnnData
 from ..genome import read_chrom_sizes
 from ._basic import FRAGMENT_PAIRED, FRAGMENT_SINGLE, _as_device_fragments
-from ._peaks import make_peak_counts
+from ._peaks import parse_regions, region_counts
 
 _GTF_ATTR = re.compile(r'\s*([^\s"]+)\s+(?:"([^"]*)"|([^;\s]+))\s*')
 
@@ -93,6 +93,29 @@ def promoter_regions(transcripts, upstream: int, downstream: int, include_gene_b
     return regs
 
 
+def _promoter_tuples(transcripts, upstream: int, downstream: int, include_gene_body: bool):
+    """The distinct regions of promoter_regions as (chrom, start, end), as
+    parse_regions reads them back from the strings (the strip of the whole
+    string removes the chromosome's leading spaces); None when a region has a
+    negative end, which the string parse rejects."""
+    seen, regs = set(), []
+    for t in transcripts:
+        left, right = t["left"] - 1, t["right"] - 1
+        if t["strand"] == "+":
+            start, end = max(left - upstream, 0), downstream + (right if include_gene_body else left)
+        elif t["strand"] == "-":
+            start, end = max((left if include_gene_body else right) - downstream, 0), right + upstream
+        else:
+            raise _lib.PanicException(f"Miss strand information for {t['transcript_id']}")
+        if end < 0:
+            return None
+        r = (str(t["chrom"]).lstrip(), start, end)
+        if r not in seen:
+            seen.add(r)
+            regs.append(r)
+    return regs
+
+
 def gene_counts(frags, chrom_sizes, transcripts, id_type="gene", upstream=2000, downstream=0,
                 include_gene_body=True, counting_strategy="paired-insertion", min_frag_size=None,
                 max_frag_size=None):
@@ -100,9 +123,11 @@ def gene_counts(frags, chrom_sizes, transcripts, id_type="gene", upstream=2000, 
     torch = _lib.require_gpu()
     L = _lib.lib()
     st = _lib.stream_ptr()
-    regions = promoter_regions(transcripts, upstream, downstream, include_gene_body)
     # counts per distinct region; the reference labels distinct region k with transcript k
-    X, _ = make_peak_counts(frags, chrom_sizes, regions, counting_strategy, min_frag_size, max_frag_size)
+    regs = _promoter_tuples(transcripts, upstream, downstream, include_gene_body)
+    if regs is None:  # the string parse raises the reference's error
+        regs = parse_regions(promoter_regions(transcripts, upstream, downstream, include_gene_body))[1]
+    X = region_counts(frags, chrom_sizes, regs, counting_strategy, min_frag_size, max_frag_size)
     n_cells = X.shape[0]
     if id_type == "transcript":
         X = DeviceCSR(X.indptr, X.indices, X.data, (n_cells, len(transcripts)))

@@ -81,6 +81,17 @@ class PeakIndex:
 def make_peak_counts(frags, chrom_sizes: dict[str, int], peaks, counting_strategy: str = "paired-insertion",
                      min_frag_size: int | None = None, max_frag_size: int | None = None):
     """(DeviceCSR cells x regions with uint32 counts, region names)."""
+    if counting_strategy not in _STRATEGY:
+        raise RuntimeError("Counting strategy must be one of 'insertion', 'fragment', or "
+                           "'paired-insertion'")  # counter.rs:25-36
+    names, regs = parse_regions(peaks)
+    return region_counts(frags, chrom_sizes, regs, counting_strategy, min_frag_size, max_frag_size), names
+
+
+def region_counts(frags, chrom_sizes: dict[str, int], regs, counting_strategy: str = "paired-insertion",
+                  min_frag_size: int | None = None, max_frag_size: int | None = None) -> DeviceCSR:
+    """DeviceCSR cells x len(regs) with uint32 counts; regs = distinct
+    (chrom, start, end) tuples, column j = regs[j]."""
     torch = _lib.require_gpu()
     L = _lib.lib()
     st = _lib.stream_ptr()
@@ -88,7 +99,6 @@ def make_peak_counts(frags, chrom_sizes: dict[str, int], peaks, counting_strateg
     if counting_strategy not in _STRATEGY:
         raise RuntimeError("Counting strategy must be one of 'insertion', 'fragment', or "
                            "'paired-insertion'")  # counter.rs:25-36
-    names, regs = parse_regions(peaks)
     idx = PeakIndex(regs, chrom_sizes, torch)
     n_cells = frags.shape[0]
     n_frag = frags.n_fragments
@@ -99,7 +109,7 @@ def make_peak_counts(frags, chrom_sizes: dict[str, int], peaks, counting_strateg
     indptr = torch.zeros(n_cells + 1, dtype=torch.int64, device=dev)
     if n_frag == 0 or n_cells == 0 or not regs:
         return DeviceCSR(indptr, torch.empty(0, dtype=torch.int32, device=dev),
-                         torch.empty(0, dtype=torch.int32, device=dev), (n_cells, len(names))), names
+                         torch.empty(0, dtype=torch.int32, device=dev), (n_cells, len(regs)))
     hits = torch.empty(n_frag, dtype=torch.int64, device=dev)
     wsb = int(L.snapgpu_peak_count_workspace_bytes(n_frag))
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
@@ -120,8 +130,7 @@ def make_peak_counts(frags, chrom_sizes: dict[str, int], peaks, counting_strateg
     out_val = torch.empty(max(nnz.value, 1), dtype=torch.int32, device=dev)
     _lib.check(L.snapgpu_peak_csr(ws.data_ptr(), wsb, T, nnz.value, n_cells, indptr.data_ptr(), out_idx.data_ptr(),
                                   out_val.data_ptr(), st))
-    X = DeviceCSR(indptr, out_idx[:nnz.value], out_val[:nnz.value], (n_cells, len(names)))
-    return X, names
+    return DeviceCSR(indptr, out_idx[:nnz.value], out_val[:nnz.value], (n_cells, len(regs)))
 
 
 def make_peak_matrix(
