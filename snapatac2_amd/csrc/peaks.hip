@@ -119,11 +119,27 @@ __global__ void emit_kernel(Index I, const int64_t* __restrict__ cell_ptr, int64
                             const uint32_t* __restrict__ pos, const uint32_t* __restrict__ val, int64_t n, int paired,
                             int strategy, uint32_t min_len, uint32_t max_len, const int64_t* __restrict__ hits_incl,
                             int64_t total, int rbits, uint64_t* __restrict__ keys) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // rows of the block's first and last fragment bound the rows of all of its
+  // fragments (the row of a fragment grows with its index); every thread
+  // reaches the barrier before any returns
+  __shared__ int64_t row_span[2];
+  const int64_t f0 = (int64_t)blockIdx.x * blockDim.x;
+  if (threadIdx.x < 2) {
+    const int64_t g = threadIdx.x == 0 ? f0 : (f0 + blockDim.x < n ? f0 + blockDim.x : n) - 1;
+    int64_t a = 0, b = n_cells;  // last row with cell_ptr[row] <= g
+    while (b - a > 1) {
+      const int64_t mid = (a + b) / 2;
+      if (cell_ptr[mid] <= g) a = mid;
+      else b = mid;
+    }
+    row_span[threadIdx.x] = a;
+  }
+  __syncthreads();
+  const int64_t f = f0 + threadIdx.x;
   if (f >= n) return;
   const Frag q = frag_at(I, pos, val, f, paired, min_len, max_len);
   if (!q.on) return;
-  int64_t lo = 0, hi = n_cells;  // last row with cell_ptr[row] <= f
+  int64_t lo = row_span[0], hi = row_span[1] + 1;  // last row with cell_ptr[row] <= f
   while (hi - lo > 1) {
     const int64_t mid = (lo + hi) / 2;
     if (cell_ptr[mid] <= f) lo = mid;
