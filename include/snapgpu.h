@@ -507,11 +507,12 @@ int snapgpu_peak_csr(const void* d_ws, int64_t ws_bytes, int64_t total, int64_t 
  * transcript), keeping the largest count of a group. snapgpu_group_max sorts
  * and groups (distinct (row, group) pairs to *h_nnz_out; synchronises), then
  * snapgpu_group_max_csr writes d_indptr[n_rows + 1], d_idx (groups, ascending
- * per row) and d_val from its workspace. */
+ * per row) and d_val from its workspace. n_groups: every d_group value is below
+ * it (0 when unknown, which sorts 32 group bits). */
 int64_t snapgpu_group_max_workspace_bytes(int64_t nnz);
 int snapgpu_group_max(const int64_t* d_indptr, int64_t n_rows, const int32_t* d_idx, const uint32_t* d_val,
-                      int64_t nnz, const uint32_t* d_group, void* d_ws, int64_t ws_bytes, int64_t* h_nnz_out,
-                      void* stream);
+                      int64_t nnz, const uint32_t* d_group, int32_t n_groups, void* d_ws, int64_t ws_bytes,
+                      int64_t* h_nnz_out, void* stream);
 int snapgpu_group_max_csr(const void* d_ws, int64_t ws_bytes, int64_t nnz, int64_t nnz_out, int64_t n_rows,
                           int64_t* d_indptr, int32_t* d_idx, uint32_t* d_val, void* stream);
 

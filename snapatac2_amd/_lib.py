@@ -94,7 +94,7 @@ SIGNATURES: dict[str, tuple] = {
                                      _i64, C.POINTER(_i64), _p]),
     "snapgpu_peak_csr": (C.c_int, [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p]),
     "snapgpu_group_max_workspace_bytes": (_i64, [_i64]),
-    "snapgpu_group_max": (C.c_int, [_p, _i64, _p, _p, _i64, _p, _p, _i64, C.POINTER(_i64), _p]),
+    "snapgpu_group_max": (C.c_int, [_p, _i64, _p, _p, _i64, _p, _i32, _p, _i64, C.POINTER(_i64), _p]),
     "snapgpu_group_max_csr": (C.c_int, [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p]),
     "snapgpu_frag_format_workspace_bytes": (_i64, [_i64]),
     "snapgpu_frag_format": (C.c_int, [_p, _i64, _p, _p, _i64, _i32, _p, _p, _i32, _p, _p, _p, _i64,

@@ -197,19 +197,34 @@ __global__ void csr_indptr_kernel(const uint64_t* __restrict__ ukey, int64_t nu,
 }
 
 // Gene level of the gene matrix: entry e of a cells x regions CSR becomes the
-// key (row << 32 | group[column]) with its count.
+// key (row << gbits | group[column]) with its count (every group below 2^gbits).
 __global__ void group_key_kernel(const int64_t* __restrict__ indptr, int64_t n_rows, const int32_t* __restrict__ idx,
                                  const uint32_t* __restrict__ val, int64_t nnz, const uint32_t* __restrict__ group,
-                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                 int gbits, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  // rows of the block's first and last entry bound the rows of all of its
+  // entries; every thread reaches the barrier before any returns
+  __shared__ int64_t row_span[2];
+  const int64_t e0 = (int64_t)blockIdx.x * blockDim.x;
+  if (threadIdx.x < 2) {
+    const int64_t g = threadIdx.x == 0 ? e0 : (e0 + blockDim.x < nnz ? e0 + blockDim.x : nnz) - 1;
+    int64_t a = 0, b = n_rows;  // last row with indptr[row] <= g
+    while (b - a > 1) {
+      const int64_t mid = (a + b) / 2;
+      if (indptr[mid] <= g) a = mid;
+      else b = mid;
+    }
+    row_span[threadIdx.x] = a;
+  }
+  __syncthreads();
+  const int64_t e = e0 + threadIdx.x;
   if (e >= nnz) return;
-  int64_t lo = 0, hi = n_rows;  // last row with indptr[row] <= e
+  int64_t lo = row_span[0], hi = row_span[1] + 1;  // last row with indptr[row] <= e
   while (hi - lo > 1) {
     const int64_t mid = (lo + hi) / 2;
     if (indptr[mid] <= e) lo = mid;
     else hi = mid;
   }
-  keys[e] = ((uint64_t)lo << 32) | group[idx[e]];
+  keys[e] = ((uint64_t)lo << gbits) | group[idx[e]];
   vals[e] = val[e];
 }
 
@@ -357,10 +372,11 @@ int64_t snapgpu_group_max_workspace_bytes(int64_t nnz) {
 }
 
 int snapgpu_group_max(const int64_t* d_indptr, int64_t n_rows, const int32_t* d_idx, const uint32_t* d_val,
-                      int64_t nnz, const uint32_t* d_group, void* d_ws, int64_t ws_bytes, int64_t* h_nnz_out,
-                      void* stream) {
+                      int64_t nnz, const uint32_t* d_group, int32_t n_groups, void* d_ws, int64_t ws_bytes,
+                      int64_t* h_nnz_out, void* stream) {
   clear_error();
-  SG_ARG(n_rows >= 1 && n_rows < (1ll << 31) && nnz >= 0 && nnz < (int64_t)INT32_MAX && h_nnz_out != nullptr,
+  SG_ARG(n_rows >= 1 && n_rows < (1ll << 31) && nnz >= 0 && nnz < (int64_t)INT32_MAX && h_nnz_out != nullptr &&
+             n_groups >= 0,
          "bad arguments");
   SG_ARG(ws_bytes >= snapgpu_group_max_workspace_bytes(nnz), "workspace too small");
   *h_nnz_out = 0;
@@ -383,10 +399,16 @@ int snapgpu_group_max(const int64_t* d_indptr, int64_t n_rows, const int32_t* d_
   void* sws = take(snapgpu_scan_workspace_bytes(nnz));
   void* tmp = p;
   size_t tmp_bytes = (size_t)(ws_bytes - (p - (char*)d_ws));
-  group_key_kernel<<<grid(nnz), kThreads, 0, st>>>(d_indptr, n_rows, d_idx, d_val, nnz, d_group, kA, vA);
+  // every group is below 2^gbits (32 when the caller gives no group count)
+  int gbits = 32;
+  if (n_groups > 0) {
+    gbits = 1;
+    while (gbits < 32 && ((uint64_t)(n_groups - 1) >> gbits) != 0) ++gbits;
+  }
+  group_key_kernel<<<grid(nnz), kThreads, 0, st>>>(d_indptr, n_rows, d_idx, d_val, nnz, d_group, gbits, kA, vA);
   SG_LAUNCH_CHECK();
-  int end_bit = 33;
-  while (end_bit < 64 && ((uint64_t)n_rows >> (end_bit - 32)) != 0) ++end_bit;
+  int end_bit = gbits + 1;  // gbits group bits, then the bits of n_rows
+  while (end_bit < 64 && ((uint64_t)n_rows >> (end_bit - gbits)) != 0) ++end_bit;
   hipcub::DoubleBuffer<uint64_t> kb(kA, kB);
   hipcub::DoubleBuffer<uint32_t> vb(vA, vB);
   SG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kb, vb, (int)nnz, 0, end_bit, st));
@@ -394,7 +416,7 @@ int snapgpu_group_max(const int64_t* d_indptr, int64_t n_rows, const int32_t* d_
   SG_LAUNCH_CHECK();
   int rc = snapgpu_scan_inclusive_i64(rid, nnz, sws, snapgpu_scan_workspace_bytes(nnz), stream);
   if (rc) return rc;
-  run_start_kernel<<<grid(nnz), kThreads, 0, st>>>(kb.Current(), nnz, rid, 32, ukey, ustart);
+  run_start_kernel<<<grid(nnz), kThreads, 0, st>>>(kb.Current(), nnz, rid, gbits, ukey, ustart);
   SG_LAUNCH_CHECK();
   // the sorted values are kept next to the run starts for snapgpu_group_max_csr
   SG_HIP(hipMemcpyAsync(uval, vb.Current(), 4 * nnz, hipMemcpyDeviceToDevice, st));

@@ -149,7 +149,7 @@ def gene_counts(frags, chrom_sizes, transcripts, id_type="gene", upstream=2000, 
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
     out_nnz = C.c_int64(0)
     _lib.check(L.snapgpu_group_max(X.indptr.data_ptr(), n_cells, X.indices.data_ptr(), X.data.data_ptr(), nnz,
-                                   d_group.data_ptr(), ws.data_ptr(), wsb, C.byref(out_nnz), st))
+                                   d_group.data_ptr(), len(genes), ws.data_ptr(), wsb, C.byref(out_nnz), st))
     idx = torch.empty(max(out_nnz.value, 1), dtype=torch.int32, device=dev)
     val = torch.empty(max(out_nnz.value, 1), dtype=torch.int32, device=dev)
     _lib.check(L.snapgpu_group_max_csr(ws.data_ptr(), wsb, nnz, out_nnz.value, n_cells, indptr.data_ptr(),
