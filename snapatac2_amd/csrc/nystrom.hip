@@ -159,8 +159,10 @@ __global__ void scale_rows_kernel(double* __restrict__ Q, int64_t ldq, int64_t n
   for (int c = 0; c < k; ++c) Q[i * ldq + c] *= r;
 }
 
-// E[i][c] = U[i][c] / sqrt(dinv[i]) / evals[c] (nystrom, embedding.rs:203-214:
-// rows times the square root of the degree, columns over the eigenvalue).
+// E[i][c] = U[i][c] * sqrt(dinv[i]) / evals[c] (nystrom, embedding.rs:203-214:
+// spectral_mf returns the degrees d = 1 / dinv (embedding.rs:189-190) and
+// nystrom multiplies row i by 1 / sqrt(d_i) = sqrt(dinv_i), then divides
+// column c by its eigenvalue).
 __global__ void landmark_scale_kernel(const double* __restrict__ U, int64_t ldu, int64_t n, int k,
                                       const double* __restrict__ dinv, const double* __restrict__ evals,
                                       double* __restrict__ E, int64_t lde) {
@@ -168,7 +170,7 @@ __global__ void landmark_scale_kernel(const double* __restrict__ U, int64_t ldu,
   if (t >= n * k) return;
   const int64_t i = t / k;
   const int c = (int)(t % k);
-  E[i * lde + c] = U[i * ldu + c] / sqrt(dinv[i]) / evals[c];
+  E[i * lde + c] = U[i * ldu + c] * sqrt(dinv[i]) / evals[c];
 }
 
 inline int waves_grid(int64_t n) { return (int)ceil_div(n > 0 ? n : 1, kThreads / 64); }

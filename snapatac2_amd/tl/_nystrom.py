@@ -122,7 +122,8 @@ def spectral_embedding_nystrom(X: DeviceCSR, n_components: int, sample_size: int
     U = U[:, torch.from_numpy(ix.copy()).to(dev)].contiguous()
     k = U.shape[1]
     ev = torch.from_numpy(evals).to(dev)
-    # nystrom (embedding.rs:203-214): rows * sqrt(deg) (= 1 / sqrt(dinv)), columns / lambda
+    # nystrom (embedding.rs:203-214): rows / sqrt(deg) with deg = 1 / dinv (embedding.rs:189-190),
+    # i.e. rows * sqrt(dinv); columns / lambda
     E = torch.empty_like(U)
     _lib.check(L.snapgpu_nystrom_landmark_scale(U.data_ptr(), k, U.shape[0], k, op.dinv.data_ptr(), ev.data_ptr(),
                                                 E.data_ptr(), k, st))

@@ -6,19 +6,11 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
+from _clusters import clustered_counts as _clustered_counts
 from oracle import spectral as osp
 
 pytestmark = pytest.mark.gpu
 
-
-def _clustered_counts(n=3000, m=1500, clusters=4, seed=0):
-    rng = np.random.default_rng(seed)
-    prof = rng.gamma(0.3, 1.0, size=(clusters, m))
-    prof /= prof.sum(axis=1, keepdims=True)
-    lab = rng.integers(0, clusters, n)
-    depth = rng.integers(80, 400, n)
-    rows = [rng.multinomial(depth[i], prof[lab[i]]) for i in range(n)]
-    return sp.csr_matrix(np.asarray(rows, dtype=np.int64))
 
 
 def _aligned_diff(A, B):
@@ -70,6 +62,8 @@ def test_spectral_nystrom_end_to_end(torch_gpu, method):
                             inplace=False)
     assert evecs.shape[0] == 2000 and evecs.shape[1] == evals.size and evals.size >= 4
     assert np.all(np.diff(evals) <= 0)
-    # the leading embedding separates the clusters as well as the exact one
+    # the leading embedding lies close to the exact one: the oracle's Nystrom on
+    # these landmarks lands 0.009 rad from it (random draw), an inverted landmark
+    # degree scaling ~0.3 rad
     ex, eu = osp.spectral(X, n_comps=6)
-    assert osp.subspace_angle(evecs[:, :3], eu[:, :3]) < 0.2
+    assert osp.subspace_angle(evecs[:, :3], eu[:, :3]) < 0.05

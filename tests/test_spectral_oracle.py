@@ -91,3 +91,54 @@ def test_orthogonalize_oracle_invariants():
     np.testing.assert_allclose(U.T @ U, np.eye(6), atol=1e-10)
     np.testing.assert_allclose((U * ev) @ U.T, (E * lam) @ E.T, atol=1e-9 * np.abs(E).max() ** 2)
     assert np.all(np.diff(ev) <= 0)
+
+
+def _nystrom_fixture():
+    from _clusters import clustered_counts
+    X = clustered_counts(3000, 1500, 4, seed=3000)
+    rows = np.sort(np.random.default_rng(7).choice(X.shape[0], 800, replace=False))
+    return X, rows
+
+
+def test_nystrom_oracle_all_landmarks_closed_form():
+    """With every cell a landmark, nystrom's product is fixed by the eigen-equation:
+    A U = U L with A = D^-1/2 (S - I) D^-1/2 gives
+    S (D^-1/2 U L^-1) = D^1/2 U + D^-1/2 U L^-1, and nystrom (embedding.rs:193-229)
+    computes q = S (U_scaled) with U_scaled = D^-1/2 U L^-1 (degrees = 1/dinv,
+    embedding.rs:189-190, rows over sqrt(degree), :205-209), then rescales each row.
+    So each output row is parallel to that closed form."""
+    from _clusters import clustered_counts
+    X = clustered_counts(600, 400, 3, seed=11)
+    rows = np.arange(X.shape[0])
+    v, q = osp.nystrom(X, rows, 5, chunk_size=250)
+    Xh, dinv, _, _ = osp.operator_parts(X)
+    ev, U = osp.eigen(Xh, dinv, 5, 0)
+    np.testing.assert_array_equal(ev, v)
+    d = 1.0 / dinv
+    ref = np.sqrt(d)[:, None] * U + (U / np.sqrt(d)[:, None]) / ev[None, :]
+    cos = np.sum(q * ref, axis=1) / (np.linalg.norm(q, axis=1) * np.linalg.norm(ref, axis=1))
+    assert np.min(cos) > 1.0 - 1e-10
+
+
+def test_nystrom_oracle_near_exact_embedding():
+    """800 of 3,000 clustered cells as landmarks, then orthogonalize, as the
+    reference's spectral() does after spectral_embedding_nystrom
+    (_embedding.py:257-265): the top-3 subspace lies within 0.05 rad of the exact
+    embedding (osp.spectral; measured 0.016). Inverting the landmark degree scaling
+    (rows times sqrt(degree) instead of over it) lands 0.32 rad away, so this pin
+    catches that mistake (next test)."""
+    X, rows = _nystrom_fixture()
+    _, eu = osp.spectral(X, n_comps=6)
+    v, q = osp.nystrom(X, rows, 6)
+    _, oq = osp.orthogonalize(v, q)
+    assert osp.subspace_angle(oq.real[:, :3], eu[:, :3]) < 0.05
+
+
+def test_nystrom_oracle_pin_rejects_inverted_scaling(monkeypatch):
+    X, rows = _nystrom_fixture()
+    _, eu = osp.spectral(X, n_comps=6)
+    monkeypatch.setattr(osp, "_landmark_scale",
+                        lambda u, v, dinv: u / np.sqrt(dinv).reshape((-1, 1)) / v.reshape((1, -1)))
+    v, q = osp.nystrom(X, rows, 6)
+    _, oq = osp.orthogonalize(v, q)
+    assert osp.subspace_angle(oq.real[:, :3], eu[:, :3]) > 0.2
