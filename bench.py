@@ -120,6 +120,7 @@ def run_pipeline(torch, chunks, cfg, timer, spmm_events, learn_steps=0):
             emb, kept = _embed(torch, evals, U, n_comps_eff)
         info["lanczos"] = st
     info["n_eigen_kept"] = kept
+    info["n_comps"] = n_comps_eff
     with timer("knn"):
         g, kst = nearest_neighbour_graph(emb, k, precision=prec, return_stats=True)
     info["knn_fallback_rows"] = kst["fallback_rows"]
@@ -133,11 +134,19 @@ def spmm_algorithmic_bytes(op, b):
     """Algorithmic bytes of one operator application on a block of b columns,
     in the form 2 [nnz e + pointers] + 4b (2n + 2m) + 8n: the canonical CSR
     figure with e = the index + value bytes actually read per entry (4 for the
-    packed panel layout, 8 for separate index and count arrays) and, as the
-    pointer arrays, whatever the layout reads instead of indptr (chunk tables
-    and row -> pair lists for the panel layout, int64 row pointers otherwise).
+    packed layouts, 8 for separate index and count arrays) and, as the pointer
+    arrays, whatever the layout reads instead of indptr (tile starts and row
+    tables for the block layout, chunk tables and row -> pair lists for the
+    panel layout, int64 row pointers otherwise).
     V read, Z written and read, Y written in b-wide rows; dinv and r per cell."""
     n, m, nnz = op.n, op.m, op.X.nnz
+    if getattr(op, "layout", "panel") == "block":
+        ptrs = 0
+        e = 0
+        for L in (op.block_csr, op.block_csc):
+            e = max(e, 4 if L.packed else 8)
+            ptrs += (L.n_blocks * L.n_panels + 1) * 8 + (L.n_blocks + 1) * 8 + L.n_rows * 8
+        return 2 * nnz * e + ptrs + 4 * b * (2 * n + 2 * m) + 8 * n
     if op.use_panels:
         ptrs = 0
         e = 0
@@ -146,6 +155,18 @@ def spmm_algorithmic_bytes(op, b):
             ptrs += (L.n_chunks + 1) * 16 + (L.n_rows + 1) * 8 + L.n_pairs * 4
         return 2 * nnz * e + ptrs + 4 * b * (2 * n + 2 * m) + 8 * n
     return 2 * (nnz * 8 + (n + 1) * 8) + 4 * b * (2 * n + 2 * m) + 8 * n
+
+
+def tile_roofline(n_frags, nnz, n_cells, seconds):
+    """Tile stage against HBM: F * 8 (positions and lengths) + nnz * 8 (columns and
+    counts written) + 2 (n + 1) * 8 (row pointers in and out) algorithmic bytes
+    over the stage's time (both passes, the per-chunk orderings and the scan)."""
+    if not seconds:
+        return None
+    alg = 8 * n_frags + 8 * nnz + 16 * (n_cells + 1)
+    gbs = alg / seconds / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "alg_bytes": int(alg), "s": seconds}
 
 
 def spmm_roofline(torch, op, events, block, workload):
@@ -164,21 +185,39 @@ def spmm_roofline(torch, op, events, block, workload):
             traffic = int(json.load(f)["traffic_bytes_per_application"])
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": (f"spectral operator application ({'panel layout: ' if op.use_panels else ''}"
+            "kernel": (f"spectral operator application ({op.layout + ' layout: ' if op.use_panels else ''}"
                        f"scale + CSC product + CSR product, {block} columns)"),
             "avg_ms": round(avg_ms, 3), "launches": len(durs), "alg_bytes": int(alg)}
+
+
+def probe_lib():
+    """libsnapgpu_probe.so (snapatac2_amd/csrc/probe/snapgpu_probe.h): the copy
+    kernel and gather probe this script measures with; not part of libsnapgpu."""
+    import ctypes as C
+    from snapatac2_amd.build import PROBE_LIB
+    L = C.CDLL(str(PROBE_LIB))
+    p, i64 = C.c_void_p, C.c_int64
+    L.snapgpu_probe_last_error.restype = C.c_char_p
+    L.snapgpu_probe_copy_bytes.argtypes = [p, p, i64, p]
+    L.snapgpu_probe_gather.argtypes = [p, i64, p, i64, p, i64, p]
+    return L
+
+
+def probe_check(L, rc):
+    if rc != 0:
+        raise RuntimeError(L.snapgpu_probe_last_error().decode())
 
 
 def gather_ceiling(torch, roof, nnz):
     """The operator's gather rate against the probe's: both sparse products
     gather one 16-float row per stored entry (2 nnz per application), and
-    snapgpu_gather_probe times the same gathers, with the same index stream and
+    snapgpu_probe_gather times the same gathers, with the same index stream and
     lane layout, alone from one L2-sized panel (32,768 rows = 2 MB) at 2^28
     uniformly random rows."""
     from snapatac2_amd import _lib
     if roof is None:
         return None
-    L = _lib.lib()
+    L = probe_lib()
     rows, count = 32768, 1 << 28
     gen = torch.Generator(device="cuda").manual_seed(0)
     idx = torch.randint(0, rows, (count,), dtype=torch.int32, device="cuda", generator=gen)
@@ -187,8 +226,8 @@ def gather_ceiling(torch, roof, nnz):
     st = _lib.stream_ptr()
 
     def run():
-        _lib.check(L.snapgpu_gather_probe(idx.data_ptr(), count, dense.data_ptr(), rows, out.data_ptr(),
-                                          out.numel(), st))
+        probe_check(L, L.snapgpu_probe_gather(idx.data_ptr(), count, dense.data_ptr(), rows, out.data_ptr(),
+                                              out.numel(), st))
     for _ in range(2):
         run()
     e0 = torch.cuda.Event(enable_timing=True)
@@ -207,27 +246,76 @@ def gather_ceiling(torch, roof, nnz):
 
 def knn_precision(torch, n, k=50, d=30):
     """Exact kNN on 10 Gaussian clusters in d dims (centres 3 N(0, I), points
-    centre + N(0, I), rows L2-normalised): BF16 MFMA candidates against FP32
-    SIMT candidates, both rescored in FP64. Returns the second of two timings of
-    each and whether the two graphs are identical."""
+    centre + N(0, I), rows L2-normalised), every path rescored in FP64:
+      bf16            BF16 MFMA candidates with kd-leaf skipping (the default)
+      bf16_all_chunks BF16 MFMA candidates over every (query leaf, point leaf)
+                      pair, like the FP32 path: the arithmetic compared alone
+      fp32            FP32 SIMT candidates over every point.
+    Second of two timings each; whether the graphs are identical; the BF16 MFMA
+    work issued by the all-chunks run (6 MFMA 16x16x32 per 16 x 16 tile, 64
+    tiles per pair of 128-point leaves) per second of the whole call, as a
+    lower bound on the candidate kernel's rate, and its fraction of the 2.5
+    PFLOP/s dense BF16 peak."""
     from snapatac2_amd.pp._knn import nearest_neighbour_graph
     gen = torch.Generator(device="cuda").manual_seed(0)
     centres = 3.0 * torch.randn(10, d, dtype=torch.float64, device="cuda", generator=gen)
     lab = torch.randint(0, 10, (n,), device="cuda", generator=gen)
     X = centres[lab] + torch.randn(n, d, dtype=torch.float64, device="cuda", generator=gen)
     X = (X / torch.linalg.norm(X, dim=1, keepdim=True)).to(torch.float32).to(torch.float64)
-    t, g = {}, {}
-    for prec in ("bf16", "fp32", "bf16", "fp32"):
+    t, g, st = {}, {}, {}
+    runs = [("bf16", "bf16", True), ("bf16_all_chunks", "bf16", False), ("fp32", "fp32", True)]
+    for name, prec, prune in runs + runs:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        g[prec] = nearest_neighbour_graph(X, k, precision=prec)
+        g[name], st[name] = nearest_neighbour_graph(X, k, precision=prec, prune=prune, return_stats=True)
         torch.cuda.synchronize()
-        t[prec] = time.perf_counter() - t0
-    same = bool(np.array_equal(g["bf16"].indices, g["fp32"].indices)
-                and np.array_equal(g["bf16"].data, g["fp32"].data))
+        t[name] = time.perf_counter() - t0
+    same = all(np.array_equal(g[a].indices, g["fp32"].indices) and np.array_equal(g[a].data, g["fp32"].data)
+               for a in ("bf16", "bf16_all_chunks"))
+    visits = st["bf16_all_chunks"]["chunk_visits"]
+    issued = visits * 64 * 6 * 2 * 16 * 16 * 32
+    tflops = issued / t["bf16_all_chunks"] / 1e12
     return {"workload": f"{n} x {d} clustered, row-normalised, k = {k}", "bf16_s": round(t["bf16"], 3),
-            "fp32_s": round(t["fp32"], 3), "bf16_speedup": round(t["fp32"] / t["bf16"], 2),
-            "identical_graphs": same}
+            "bf16_all_chunks_s": round(t["bf16_all_chunks"], 3), "fp32_s": round(t["fp32"], 3),
+            "bf16_speedup": round(t["fp32"] / t["bf16"], 2),
+            "bf16_speedup_same_scan": round(t["fp32"] / t["bf16_all_chunks"], 2),
+            "bf16_chunk_visit_frac": round(st["bf16"]["chunk_visits"] / max(1, st["bf16"]["chunk_pairs"]), 4),
+            "mfma": {"issued_tflops": round(tflops, 1), "peak_tflops": 2500.0,
+                     "frac": round(tflops / 2500.0, 4),
+                     "note": "bf16 MFMA flops issued by the all-chunks run over its whole call time "
+                             "(three-term split: 6 MFMA per 16 x 16 tile of 32-wide rows)"},
+            "identical_graphs": bool(same)}
+
+
+def embedding_quality(torch, pinfo, g_learned, g_plain, cfg, steps):
+    """Learned against IDF feature weights, judged on what the generator knows:
+    each cell's type. kNN purity (the fraction of a cell's k neighbours of its
+    own type) of the timed pipeline's graph (weights after one ascent step), of
+    the plain path's (IDF), and, when steps > 0, of a graph after `steps` ascent
+    steps from IDF (untimed), with that run's eigengap history (FP64 gaps)."""
+    from snapatac2_amd import synth
+    from snapatac2_amd.pp._knn import nearest_neighbour_graph
+    from snapatac2_amd.tl._adjoint import tune_feature_weights
+    n, k = cfg[0], cfg[3]
+    types = synth.cell_types(n).cpu().numpy()
+    out = {"k": k, "purity_idf": round(knn_purity(g_plain, types, k), 5),
+           "purity_learned_1_step": round(knn_purity(g_learned, types, k), 5)}
+    if steps > 0:
+        op = pinfo["op"]
+        t0 = time.perf_counter()
+        tr = tune_feature_weights(op, pinfo["n_comps"], steps=steps, seed=0)
+        emb, _ = _embed(torch, tr.evals, tr.vectors, pinfo["n_comps"])
+        g = nearest_neighbour_graph(emb, k, precision=cfg[4])
+        out[f"purity_learned_{steps}_steps"] = round(knn_purity(g, types, k), 5)
+        out[f"gaps_{steps}_steps"] = [float(h[0]) for h in tr.history]
+        out[f"ascent_{steps}_steps_s"] = round(time.perf_counter() - t0, 2)
+    return out
+
+
+def knn_purity(g, types, k):
+    """Fraction of kNN neighbours that share the cell's generator type."""
+    idx = np.asarray(g.indices).reshape(-1, k)
+    return float((types[idx] == types[:, None]).mean())
 
 
 def _barcodes(n):
@@ -453,11 +541,71 @@ def gene_bench(torch, frags, cpu_cells):
     return out
 
 
-def cpu_baseline(torch, chunks, cfg, emb, sel_nnz_full):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _selected_matrix(torch, n, n_feat):
+    """The selected count matrix of n generator cells (seed 0), as the pipeline
+    builds it: 500 bp tiles, default exclusions, top n_feat features."""
+    from snapatac2_amd import synth
+    from snapatac2_amd.genome import synthetic_genome
+    from snapatac2_amd.pp import make_tile_matrix_chunked
+    from snapatac2_amd.pp._basic import _column_sums, _find_most_accessible_features
+    from snapatac2_amd.tl._operator import select_columns
+    X, _ = make_tile_matrix_chunked(list(synth.iter_chunks(n, seed=0)), synthetic_genome(), 500, EXCLUDE)
+    sel = np.sort(_find_most_accessible_features(_column_sums(X), 0.005, 0.005, n_feat))
+    return select_columns(X, sel)
+
+
+def arpack_on_device(torch, n, cfg, parity: bool):
+    """The reference's eigensolver call (scipy eigsh, which='LM', v0 =
+    RandomState(0).rand(n), default ncv/tol; embedding.rs:155-171) on the FP64
+    operator of n generator cells, its products on the GPU through torch
+    (oracle.spectral.eigen_device): the matvec count the CPU reference would
+    make at that size. With parity, also k + 1 pairs compared against this
+    package's solver (oracle.spectral.eigen_parity)."""
+    from oracle import adjoint as oadj
+    from oracle import spectral as osp
+    from snapatac2_amd.tl._lanczos import eigsh_lm
+    from snapatac2_amd.tl._operator import SpectralOperator
+    Xs = _selected_matrix(torch, n, cfg[1])
+    op = SpectralOperator(Xs)
+    T = oadj.ChunkedCounts(Xs.indptr, Xs.indices, Xs.data, Xs.shape, chunk_entries=1 << 26)
+    k = min(cfg[2], n - 2) + (1 if parity else 0)
+    t0 = time.perf_counter()
+    ref_evals, ref_U, nmv = osp.eigen_device(T, op.w, k, seed=0)
+    t_arpack = time.perf_counter() - t0
+    out = {"cells": n, "k": k, "matvecs": nmv, "arpack_s": round(t_arpack, 2)}
+    if parity:
+        evals, U, st = eigsh_lm(op, k, seed=0)
+        ix = np.argsort(evals, kind="stable")[::-1].copy()
+        evals = np.asarray(evals)[ix]
+        U = U[:, torch.from_numpy(ix).to(U.device)]
+        AU = T.apply(op.w, U)
+        lam = torch.from_numpy(evals).to(U.device)
+        res = torch.linalg.norm(AU - U * lam, dim=0).cpu().numpy()
+        par = osp.eigen_parity(evals, U.cpu().numpy(), ref_evals, ref_U, res, k - 1)
+        par["separated"] = {str(j): round(v, 12) for j, v in par["separated"].items()}
+        par["max_residual"] = float(res[:k - 1].max())
+        out["parity"] = par
+    return out
+
+
+def cpu_baseline(torch, chunks, cfg, emb, sel_nnz_full, parity_cells=100_000):
     """Reference algorithm (oracle restatements) timed on this host's cores on a
-    bounded sample, extrapolated to the workload's cell count."""
+    bounded sample, extrapolated to the workload's cell count. The spectral
+    stage's matvec count is the reference eigsh call's own, measured at the
+    sample and (with device matvecs) at 20,000 and parity_cells cells, and
+    extrapolated to the workload's size by a fit in log(cells)."""
     import scipy.sparse as sp
-    from oracle import knn as oknn
     from oracle import spectral as osp
     from oracle import tile as otile
     from snapatac2_amd.genome import synthetic_genome
@@ -475,7 +623,7 @@ def cpu_baseline(torch, chunks, cfg, emb, sel_nnz_full):
     t0 = time.perf_counter()
     Xh = otile.tile_matrix(cp.astype(np.uint64), pos, ln, cs, 500, ["chrY"])
     t_tile = (time.perf_counter() - t0) / ns * n_cells
-    # 2) spectral: ARPACK matvec count on the sample, matvec time scaled by nnz
+    # 2) spectral: ARPACK on the sample (host), matvec time scaled by nnz
     count = np.ravel(Xh.sum(axis=0)).astype(np.float64)
     sel = np.sort(_find_most_accessible_features(count, 0.005, 0.005, cfg[1]))
     Xsel = Xh[:, sel]
@@ -497,9 +645,27 @@ def cpu_baseline(torch, chunks, cfg, emb, sel_nnz_full):
     for _ in range(5):
         f(v)
     t_mv_sample = (time.perf_counter() - t0) / 5
+    counts = {ns: matvecs[0]}
+    parity = None
+    for nc in sorted({min(20_000, n_cells), min(parity_cells, n_cells)}):
+        if nc <= ns:
+            continue
+        log(f"[bench] cpu baseline: reference eigsh call at {nc} cells (device FP64 matvecs)")
+        r = arpack_on_device(torch, nc, cfg, parity=(nc == parity_cells))
+        counts[nc] = r["matvecs"]
+        if "parity" in r:
+            parity = dict(r["parity"], arpack_matvecs=r["matvecs"], arpack_s=r["arpack_s"], cells=nc)
+        torch.cuda.empty_cache()
+    xs = np.log(np.asarray(sorted(counts), dtype=np.float64))
+    ys = np.asarray([counts[c] for c in sorted(counts)], dtype=np.float64)
+    if xs.size >= 2:
+        b, a = np.polyfit(xs, ys, 1)
+        mv_full = max(float(ys.max()), a + b * np.log(n_cells))
+    else:
+        mv_full = float(ys[0])
     # scipy's CSR matvec is single threaded and linear in nnz: scale by the
     # selected nnz of the full matrix (measured on the GPU run)
-    t_spec = matvecs[0] * t_mv_sample * (sel_nnz_full / max(Xhat.nnz, 1))
+    t_spec = mv_full * t_mv_sample * (sel_nnz_full / max(Xhat.nnz, 1))
     # 3) kNN: exact kd-tree (scipy cKDTree, all workers) on the full embedding;
     # the query time of a row sample is scaled to every row
     from scipy.spatial import cKDTree
@@ -514,13 +680,18 @@ def cpu_baseline(torch, chunks, cfg, emb, sel_nnz_full):
     t_knn = t_build + (time.perf_counter() - t0) / q * E.shape[0]
     total = t_tile + t_spec + t_knn
     return {"value": round(n_cells / total, 3), "unit": "cells/s", "cores": cores, "kind": "port",
+            "cpu_model": _cpu_model(), "host_logical_cpus": os.cpu_count(),
             "sample": (f"{ns} cells: tile (C++ std::map per row, OpenMP) timed on the sample; "
-                       f"spectral = ARPACK matvec count on the sample ({matvecs[0]}) x scipy "
-                       f"matvec time scaled to the full selected nnz ({sel_nnz_full}); kNN = "
-                       f"cKDTree build on all rows + query time of {q} rows scaled to all rows; "
-                       f"assumes the sample's ARPACK matvec count holds at {n_cells} cells"),
+                       f"spectral = the reference eigsh call's matvec count, measured at "
+                       f"{sorted(counts)} cells ({[counts[c] for c in sorted(counts)]}) and fitted in "
+                       f"log(cells) to {round(mv_full)} at {n_cells} cells (k + 1 pairs at the "
+                       f"{parity_cells}-cell parity run), x the scipy matvec time of "
+                       f"the sample scaled to the full selected nnz ({sel_nnz_full}); kNN = cKDTree "
+                       f"build on all rows + query time of {q} rows scaled to all rows"),
             "stage_s": {"tile": round(t_tile, 2), "spectral": round(t_spec, 2), "knn": round(t_knn, 2),
-                        "eigsh_sample_s": round(t_eig_sample, 2)}}
+                        "eigsh_sample_s": round(t_eig_sample, 2)},
+            "arpack_matvecs": {str(c): counts[c] for c in sorted(counts)},
+            "eigen_parity": parity}
 
 
 def _knn_sample(E, qi, k):
@@ -550,6 +721,13 @@ def main():
                     help="skip the cell x peak and cell x gene matrix measurements")
     ap.add_argument("--import-only", action="store_true",
                     help="run only the fragment-import measurement (its own JSON line)")
+    ap.add_argument("--layout", default=None, choices=["block", "panel"],
+                    help="operator layout of the large sparse products (default: the package default)")
+    ap.add_argument("--parity-cells", type=int, default=100_000,
+                    help="cells of the reference-eigsh run in the CPU baseline leg, whose eigenpairs "
+                         "are also compared with this solver's")
+    ap.add_argument("--ascent-steps", type=int, default=4,
+                    help="eigengap-ascent steps of the untimed learned-weights quality check (0: skip it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one copy kernel of MIB MiB (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -559,13 +737,17 @@ def main():
         sys.exit(2)
     import torch
     from snapatac2_amd import synth
+    from snapatac2_amd.tl import _operator
+    if args.layout:
+        _operator.DEFAULT_TILED = args.layout
     torch.cuda.set_device(0)
     if args.copy_calibration:
         from snapatac2_amd import _lib
         nb = args.copy_calibration << 20
         src = torch.empty(nb, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
-        _lib.check(_lib.lib().snapgpu_copy_bytes(src.data_ptr(), dst.data_ptr(), nb, _lib.stream_ptr()))
+        P = probe_lib()
+        probe_check(P, P.snapgpu_probe_copy_bytes(src.data_ptr(), dst.data_ptr(), nb, _lib.stream_ptr()))
         torch.cuda.synchronize()
         log(f"[bench] copy calibration: {nb} bytes read, {nb} bytes written")
         del src, dst
@@ -609,7 +791,7 @@ def main():
     ms = float(np.mean(times)) * 1e3
     value = n_cells / (ms / 1e3)
     stages = {k: round(v, 3) for k, v in timer.t.items()}
-    plain = None
+    plain = quality = None
     nys_X = info["op"].X if "op" in info else None
     if cfg[5] > 0:
         # the plain GPU path (IDF weights, no ascent) on the same resident input
@@ -617,15 +799,17 @@ def main():
         del info["op"]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        pinfo, _, _ = run_pipeline(torch, chunks, cfg, ptimer, None, learn_steps=0)
+        pinfo, _, g_plain = run_pipeline(torch, chunks, cfg, ptimer, None, learn_steps=0)
         torch.cuda.synchronize()
         pdt = time.perf_counter() - t0
         nys_X = pinfo["op"].X  # the selected matrix, for the Nystrom measurement
-        del pinfo["op"]
         plain = {"ms_per_step": round(pdt * 1e3, 1), "cells_per_s": round(n_cells / pdt, 1),
                  "stages_s": {k: round(v, 3) for k, v in ptimer.t.items()},
                  "lanczos": pinfo["lanczos"]}
         log(f"[bench] plain path (IDF weights): {pdt:.3f}s")
+        quality = embedding_quality(torch, pinfo, g, g_plain, cfg, args.ascent_steps)
+        log(f"[bench] embedding quality: {quality}")
+        del pinfo["op"]
     log(f"[bench] lanczos {info['lanczos']}  nnz tile={info['tile_nnz']} sel={info['selected_nnz']}")
     # recall spot check of the kNN output against exact FP64 neighbours
     E = emb.cpu().numpy()
@@ -654,7 +838,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline ...")
-        cpu = cpu_baseline(torch, chunks, cfg, emb, info["selected_nnz"])
+        cpu = cpu_baseline(torch, chunks, cfg, emb, info["selected_nnz"], args.parity_cells)
     out = {
         "metric": "cells/s tile+spectral+kNN at 1M cells; SpMM HBM GB/s; kNN recall@50 vs FP64",
         "value": round(value, 1),
@@ -680,6 +864,7 @@ def main():
         "peak_matrix": peaks,
         "gene_matrix": genes,
         "cpu_baseline": cpu,
+        "tile_roofline": tile_roofline(n_frags, info["tile_nnz"], n_cells, stages.get("tile")),
         "spmm_gbps": roof["achieved"] if roof else None,
         "knn_recall_sample": recall,
         "knn_fallback_rows": info["knn_fallback_rows"],
@@ -687,6 +872,8 @@ def main():
         "stages_s": stages,
         "lanczos": info["lanczos"],
         "eigengap": info.get("eigengap"),
+        "embedding_quality": quality,
+        "eigen_parity": cpu.get("eigen_parity") if cpu else None,
         "plain_path": plain,
     }
     if cpu:

@@ -26,7 +26,9 @@ enum snapgpu_status {
   SNAPGPU_ERR_ARG = 1,      /* bad argument (shape, enum value, null pointer) */
   SNAPGPU_ERR_HIP = 2,      /* HIP runtime failure                            */
   SNAPGPU_ERR_DATA = 3,     /* input breaks an invariant the reference panics on */
-  SNAPGPU_ERR_CAPACITY = 4  /* input exceeds a documented kernel limit        */
+  SNAPGPU_ERR_CAPACITY = 4, /* input exceeds a documented kernel limit        */
+  SNAPGPU_ERR_UNSORTED = 5  /* a row of fragments is not sorted by position:
+                               sort it (snapgpu_frag_sort_rows), call again  */
 };
 
 /* Counting strategies of the tile matrix.
@@ -41,20 +43,6 @@ enum snapgpu_strategy {
 
 const char* snapgpu_last_error(void);
 int snapgpu_abi_version(void);
-
-/* Streaming device-to-device copy of `bytes` (multiple of 16, 16-byte aligned
- * pointers). A kernel of known traffic, used to calibrate the hardware byte
- * counters that profiles report. */
-int snapgpu_copy_bytes(const void* d_src, void* d_dst, int64_t bytes, void* stream);
-
-/* Gather-rate probe (no reference counterpart; a measurement aid): reads one
- * 16-float row of d_dense[rows][16] per entry of d_idx (row ids < rows), with
- * the access pattern of the operator's sparse products (16 lanes per row, 64
- * indices per group batch, eight rows in flight per lane). Timed by the caller;
- * with an L2-sized d_dense it gives the gather ceiling of those products.
- * d_out: 4096 * 256 floats (per-thread sums, so the loads stay live). */
-int snapgpu_gather_probe(const uint32_t* d_idx, int64_t count, const float* d_dense, int64_t rows,
-                         float* d_out, int64_t out_len, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Prefix sums                                                              */
@@ -100,6 +88,20 @@ int64_t snapgpu_tile_order_workspace_bytes(int64_t n_cells);
 int snapgpu_tile_order(const uint64_t* d_cell_ptr, int64_t n_cells, uint32_t* d_order, void* d_ws,
                        int64_t ws_bytes, void* stream);
 
+/* Sorts the fragments of every row by (position, value) in place, the order
+ * import_fragments stores them in (import.rs:208). The tile passes need rows
+ * whose positions do not decrease and return SNAPGPU_ERR_UNSORTED (before any
+ * output is used) when one does; the reference's per-row BTreeMap accepts any
+ * order (data_iter.rs:362-411), so sorting and calling again gives its result.
+ * d_cell_ptr[n_cells + 1]: row pointers into d_pos / d_len, all within
+ * [0, n_fragments], n_fragments < 2^31; d_len may be NULL (positions only).
+ * Workspace:
+ * snapgpu_frag_sort_rows_workspace_bytes(n_cells, n_fragments). */
+int64_t snapgpu_frag_sort_rows_workspace_bytes(int64_t n_cells, int64_t n_fragments);
+int snapgpu_frag_sort_rows(const uint64_t* d_cell_ptr, int64_t n_cells, int64_t n_fragments,
+                           uint32_t* d_pos, uint32_t* d_len, void* d_ws, int64_t ws_bytes,
+                           void* stream);
+
 /* Pass 1: number of stored (bin, count) entries of each cell.
  *   d_cell_ptr[n_cells+1]: fragment row pointers of obsm['fragment_paired'|
  *     'fragment_single'] (import.rs:117-159);
@@ -112,6 +114,7 @@ int snapgpu_tile_order(const uint64_t* d_cell_ptr, int64_t n_cells, uint32_t* d_
  *   d_status[8]: int32 device scratch for error reporting;
  *   *level: in = first capacity level to try (0), out = level that succeeded
  *     (pass it to snapgpu_tile_fill).
+ * Rows must be sorted by position (SNAPGPU_ERR_UNSORTED otherwise).
  * Synchronises `stream` once per level tried. */
 int snapgpu_tile_count(const uint64_t* d_cell_ptr, const uint32_t* d_pos,
                        const uint32_t* d_len, int64_t n_cells, const void* d_table,
@@ -165,6 +168,12 @@ int snapgpu_synth_counts(uint64_t seed, int64_t first_cell, int64_t n_cells, dou
                          double sigma, int64_t min_frags, int64_t max_frags, int64_t* d_counts,
                          void* stream);
 
+/* Cell type (0 .. n_types-1) of cells first_cell .. first_cell+n_cells-1: the
+ * type whose peak activity snapgpu_synth_fragments draws the cell's peak
+ * fragments from (the generator's ground truth, for judging embeddings). */
+int snapgpu_synth_cell_types(uint64_t seed, int64_t first_cell, int64_t n_cells, int32_t n_types,
+                             int32_t* d_types, void* stream);
+
 int64_t snapgpu_synth_workspace_bytes(int64_t n_frags);
 
 /* Fragments of a chunk of cells (< 2^20 cells, < 2^31 fragments), sorted per
@@ -188,7 +197,8 @@ int snapgpu_synth_fragments(uint64_t seed, int64_t first_cell, int64_t n_cells,
 
 int64_t snapgpu_knn_workspace_bytes(int64_t n, int32_t d, int32_t k);
 
-/* d_X: n x d row-major FP64 (leading dimension ld), d <= 30, 1 <= k <= min(100, n-1).
+/* d_X: n x d row-major FP64 (leading dimension ld), d >= 1, 1 <= k <= n-1.
+ * For d <= 30 and k <= 100:
  * mode 0: FP32 SIMT candidate distances; mode 1: BF16 MFMA (hi/lo split, FP32
  * accumulation). Both rescore candidates in exact FP64 and prove the result
  * with a certificate (failing rows are recomputed exactly), so the neighbour
@@ -201,7 +211,11 @@ int64_t snapgpu_knn_workspace_bytes(int64_t n, int32_t d, int32_t k);
  * farther than every candidate threshold of a 128-query workgroup;
  * *n_chunk_visits (may be NULL) receives the (workgroup, chunk) pairs processed,
  * out of ceil(n/128)^2. flags bit 0: send every row straight to the exact scan
- * (testing); bit 1: visit every chunk (no skipping). Synchronises `stream`. */
+ * (testing); bit 1: visit every chunk (no skipping).
+ * Wider rows (d > 30) or more neighbours (k > 100), in either mode: exact FP64
+ * distances from every query to every point (O(n^2 d)) and a stable sort of
+ * each query's (distance, index) pairs; *n_exact = n, the other counters 0.
+ * Synchronises `stream`. */
 int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, int32_t mode,
                 int32_t* d_nbr, double* d_dist, void* d_workspace, int64_t workspace_bytes,
                 int32_t flags, int64_t* n_fallback, int64_t* n_exact, int64_t* n_chunk_visits,
@@ -335,6 +349,64 @@ int snapgpu_spectral_apply_panel(const snapgpu_panel_layout* csr, const snapgpu_
                                  const float* d_r32, const float* d_w2, const double* d_dinv,
                                  const double* d_V, int64_t ldv, int32_t b, double* d_Y,
                                  int64_t ldy, void* d_ws, int64_t ws_bytes, void* stream);
+
+/* Block-tiled layout (the faster form of the same products for b <= 16). One
+ * workgroup owns a block of output rows (rows ordered by decreasing entry
+ * count, blocks of about equal work) and keeps their sums in LDS; the dense
+ * operand is swept in panels of snapgpu_block_panel_rows() rows, each read
+ * once per block into LDS, from which every entry of the tile gathers its row.
+ * A row's entries inside one panel are cut into pieces of at most 16, each
+ * with its own accumulator slot, so the 256 four-lane groups of a workgroup
+ * share a tile's entries evenly and every slot has one writer: fixed summation
+ * order. Build: snapgpu_block_plan (row order, pieces, blocks; its workspace
+ * keeps them), then allocate, then snapgpu_block_build.
+ *   ent / val      n_entries uint32 (val only when not packed: counts >= 1024
+ *                  or float values)
+ *   tile_start     n_blocks * n_panels + 1 int64 (tile = (block, panel))
+ *   row0           n_blocks + 1 int32 (first out position of each block)
+ *   out_row        n_rows int32 (row of each out position)
+ *   vslot          n_rows int32 (first slot of each out position in its block)
+ *   nslots         n_blocks int32 */
+typedef struct snapgpu_block_layout {
+  const uint32_t* ent;
+  const uint32_t* val;
+  const int64_t* tile_start;
+  const int32_t* row0;
+  const int32_t* out_row;
+  const int32_t* vslot;
+  const int32_t* nslots;
+  int64_t n_blocks;
+  int64_t n_panels;
+  int64_t n_rows;
+  int64_t dense_rows;
+  int64_t n_entries;
+  int32_t packed;
+  int32_t value_type;
+} snapgpu_block_layout;
+
+int32_t snapgpu_block_panel_rows(void);
+int64_t snapgpu_block_plan_workspace_bytes(int64_t nrows);
+/* Orders the rows, sizes their pieces and forms about target_blocks blocks;
+ * returns the block count, the size of the build's piece table and whether the
+ * entries pack. Synchronises. */
+int snapgpu_block_plan(const int64_t* d_ptr, const int32_t* d_idx, const uint32_t* d_val, int64_t nrows,
+                       int64_t dense_rows, int64_t nnz, int32_t value_type, int64_t target_blocks, void* d_ws,
+                       int64_t ws_bytes, int64_t* n_blocks, int64_t* table_size, int32_t* packed, void* stream);
+int64_t snapgpu_block_build_workspace_bytes(int64_t table_size);
+/* Fills the layout arrays from the plan (d_plan_ws as left by snapgpu_block_plan).
+ * Synchronises. */
+int snapgpu_block_build(const int64_t* d_ptr, const int32_t* d_idx, const uint32_t* d_val, int64_t nrows,
+                        int64_t dense_rows, int64_t nnz, int32_t packed, const void* d_plan_ws, int64_t plan_ws_bytes,
+                        int64_t n_blocks, int64_t table_size, void* d_ws, int64_t ws_bytes, uint32_t* d_ent,
+                        uint32_t* d_val_out, int64_t* d_tile_start, int32_t* d_row0, int32_t* d_out_row,
+                        int32_t* d_vslot, int32_t* d_nslots, void* stream);
+/* Y = A V as snapgpu_spectral_apply (b <= 16), over the two block layouts (csr:
+ * rows = cells, dense_rows = features; csc: rows = features, dense_rows = cells). */
+int64_t snapgpu_spectral_apply_block_workspace_bytes(int64_t n, int64_t m);
+int snapgpu_spectral_apply_block(const snapgpu_block_layout* csr, const snapgpu_block_layout* csc,
+                                 const float* d_r32, const float* d_w2, const double* d_dinv, const double* d_V,
+                                 int64_t ldv, int32_t b, double* d_Y, int64_t ldy, void* d_ws, int64_t ws_bytes,
+                                 void* stream);
 
 /* FP64 tall-skinny products for the block Lanczos basis (the work ARPACK's
  * dsaupd does inside eigsh, embedding.rs:164). Row-major operands.

@@ -9,11 +9,16 @@ Pinning status per function is recorded in each module's docstring.
 """
 from pathlib import Path
 
+import os
+
 ORACLE_DIR = Path(__file__).resolve().parent
-LIB_PATH = ORACLE_DIR / "_build" / "liboracle.so"
+# SNAPGPU_ORACLE_SANITIZED=1 selects the AddressSanitizer/UBSan build of the same
+# sources (make sanitize; load it with libasan preloaded)
+_SAN = os.environ.get("SNAPGPU_ORACLE_SANITIZED") == "1"
+LIB_PATH = ORACLE_DIR / "_build" / ("liboracle_san.so" if _SAN else "liboracle.so")
 
 
 def build() -> Path:
     import subprocess
-    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)] + (["sanitize"] if _SAN else []), check=True)
     return LIB_PATH

@@ -14,13 +14,17 @@ Three independent FP64 restatements of d<G, A(w) V>/dw:
 and, for full-size checks, the forward and the closed form through torch's
 sparse products on any device (TorchCounts, apply_tensors,
 closed_form_grad_tensors; torch.sparse autograd materialises a dense n x m
-gradient there, so it is used at small sizes only).
+gradient there, so it is used at small sizes only), and for matrices too large
+for those COO copies the forward by row chunks with torch gathers and
+index_add_ (ChunkedCounts, with idf_from_df).
 
 Pinned by tests/test_adjoint_oracle.py: the three (and the torch-tensor
 forms) agree with each other, the
 forward equals oracle/spectral.py's operator, dV = A G (symmetry), the scale
 invariance <dw, w> = 0 (A(c w) = A(w): a dropped row-norm term breaks it), and
-Hellmann-Feynman against finite differences of dense eigenvalues.
+Hellmann-Feynman against finite differences of dense eigenvalues;
+ChunkedCounts.apply equals the forward over several chunks, and idf_from_df
+equals oracle/spectral.py's idf, including its all-equal and clamp cases.
 """
 from __future__ import annotations
 
@@ -190,3 +194,99 @@ def closed_form_grad(X, w, V, G) -> np.ndarray:
     kappa = np.where(ne, alpha * a * u ** 3 + gamma * u ** 2 * (d + 1.0) + u ** 3 * psi, 0.0)
     sigma = np.asarray(Xc.multiply(Xc).T @ kappa).ravel()
     return w * (2.0 * h + 2.0 * e * phi - sigma)
+
+
+class ChunkedCounts:
+    """X (a device CSR of integer counts: int64 row pointers, int32/int64
+    columns, uint32 counts in an int32 tensor) for FP64 products at sizes where
+    TorchCounts' COO copies do not fit: the entries are visited in row chunks,
+    each turned into (row, column, value) tensors, and the products are torch
+    gathers and index_add_ (torch's kernels, not this project's). Same formulas
+    as TorchCounts (embedding.rs:133-162 with the weighting of :314-326)."""
+
+    def __init__(self, indptr, indices, counts, shape, chunk_entries: int = 1 << 28):
+        import torch
+        self.torch = torch
+        self.n, self.m = int(shape[0]), int(shape[1])
+        self.indptr, self.indices, self.counts = indptr, indices, counts
+        ip = indptr.cpu().numpy()
+        bounds = [0]
+        while bounds[-1] < self.n:
+            r0 = bounds[-1]
+            r1 = int(np.searchsorted(ip, ip[r0] + chunk_entries, side="right")) - 1
+            bounds.append(min(self.n, max(r1, r0 + 1)))
+        self.bounds = bounds
+        self.nonempty = (indptr[1:] > indptr[:-1])
+
+    def chunks(self, squared: bool = False):
+        torch = self.torch
+        for r0, r1 in zip(self.bounds[:-1], self.bounds[1:]):
+            a, b = int(self.indptr[r0].item()), int(self.indptr[r1].item())
+            lens = torch.diff(self.indptr[r0:r1 + 1])
+            rows = torch.repeat_interleave(torch.arange(r0, r1, device=lens.device), lens)
+            cols = self.indices[a:b].long()
+            v = (self.counts[a:b].long() & 0xFFFFFFFF).double()
+            yield rows, cols, (v * v if squared else v)
+
+    def mm(self, B, squared: bool = False):
+        """X B (or (X o X) B) for a dense m x k (or m-vector) B."""
+        torch = self.torch
+        vec = B.dim() == 1
+        Bm = B[:, None] if vec else B
+        out = torch.zeros((self.n, Bm.shape[1]), dtype=torch.float64, device=Bm.device)
+        for rows, cols, v in self.chunks(squared):
+            out.index_add_(0, rows, v[:, None] * Bm[cols])
+        return out[:, 0] if vec else out
+
+    def rmm(self, B, squared: bool = False):
+        """X^T B for a dense n x k (or n-vector) B."""
+        torch = self.torch
+        vec = B.dim() == 1
+        Bm = B[:, None] if vec else B
+        out = torch.zeros((self.m, Bm.shape[1]), dtype=torch.float64, device=Bm.device)
+        for rows, cols, v in self.chunks(squared):
+            out.index_add_(0, cols, v[:, None] * Bm[rows])
+        return out[:, 0] if vec else out
+
+    def df(self):
+        """Stored entries per column (idf's document frequency)."""
+        torch = self.torch
+        out = torch.zeros(self.m, dtype=torch.int64, device=self.indptr.device)
+        for _, cols, _ in self.chunks():
+            out += torch.bincount(cols, minlength=self.m)
+        return out
+
+    def col_sums(self):
+        torch = self.torch
+        out = torch.zeros(self.m, dtype=torch.float64, device=self.indptr.device)
+        for _, cols, v in self.chunks():
+            out.index_add_(0, cols, v)
+        return out
+
+    def scales(self, w):
+        """(u, d, a): row scales 1/||x o w||, degrees d = X~ X~^T 1 - 1 and |d|^-1/2."""
+        torch = self.torch
+        nrm = self.mm(w * w, squared=True)
+        u = torch.where(nrm > 0, nrm.clamp_min(1e-300).rsqrt(), torch.zeros_like(nrm))
+        e = self.rmm(u)
+        d = u * self.mm(w * w * e) - 1.0
+        a = torch.where(self.nonempty, d.abs().rsqrt(), torch.zeros_like(d))
+        return u, d, a
+
+    def apply(self, w, V):
+        """A(w) V in FP64."""
+        u, d, a = self.scales(w)
+        r = (u * a)[:, None]
+        return r * self.mm((w * w)[:, None] * self.rmm(r * V)) - V / d[:, None]
+
+
+def idf_from_df(df, n: int):
+    """idf (embedding.rs:269-286) from the document frequencies (torch, FP64):
+    all equal -> 1; else df 0 -> 1, df n -> n - 1, w = ln(n / df)."""
+    import torch
+    if bool((df == df[0]).all()):
+        return torch.ones(df.shape[0], dtype=torch.float64, device=df.device)
+    d = df.double().clone()
+    d[df == 0] = 1.0
+    d[df == n] = n - 1.0
+    return torch.log(n / d)

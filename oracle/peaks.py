@@ -122,3 +122,24 @@ def peak_matrix_cpp(cell_ptr, pos, val, paired: bool, chrom_sizes: dict, peaks, 
     h.oracle_peak_run(*args, indptr.ctypes.data, idx.ctypes.data if idx.size else None,
                       dat.ctypes.data if dat.size else None)
     return indptr, idx.astype(np.int64), dat
+
+
+def intersect_bed(names, regs) -> np.ndarray:
+    """intersect_bed (snapatac2-python/src/utils.rs:320-336) by brute force:
+    names[i] ("chrom:start-end", GenomicRange::from_str) is True when some BED
+    region (chrom, start, end) of `regs` shares a base with it on the same
+    chromosome (half-open: r.start < end and start < r.end). Pinned by the hand
+    cases of tests/test_regions_host.py (touching ends, containment, empty
+    queries, other chromosomes)."""
+    by_chrom: dict[str, list[tuple[int, int]]] = {}
+    for c, s, e in regs:
+        by_chrom.setdefault(c, []).append((s, e))
+    arrs = {c: np.asarray(v, dtype=np.int64).reshape(-1, 2) for c, v in by_chrom.items()}
+    out = np.zeros(len(names), dtype=bool)
+    for i, nm in enumerate(names):
+        chrom, rng = str(nm).strip().rsplit(":", 1)
+        a, b = (int(x) for x in rng.split("-", 1))
+        r = arrs.get(chrom)
+        if r is not None:
+            out[i] = bool(np.any((r[:, 0] < b) & (a < r[:, 1])))
+    return out

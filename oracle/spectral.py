@@ -165,3 +165,73 @@ def orthogonalize(evals, evecs):
     np.divide(evecs_new, sigma.reshape((-1, 1)), out=evecs_new)
     evecs_new = evecs @ V @ evecs_new
     return (evals_new, evecs_new)
+
+
+def subspace_sin(U: np.ndarray, V: np.ndarray) -> float:
+    """Sine of the largest principal angle between span(U) and span(V), as the
+    norm of V's orthonormal basis minus its projection on U's (accurate for
+    small angles, where arccos of the cosines is not)."""
+    qu, _ = np.linalg.qr(U)
+    qv, _ = np.linalg.qr(V)
+    r = qv - qu @ (qu.T @ qv)
+    return float(np.linalg.norm(r, 2))
+
+
+def eigen_device(T, w, k: int, seed: int = 0):
+    """eigen() of embedding.rs:155-171 — scipy eigsh with which='LM',
+    v0 = RandomState(seed).rand(n) and its default ncv and tol — on the FP64
+    operator A(w) of the counts held by T (oracle.adjoint.ChunkedCounts), whose
+    products run on T's device through torch: the reference's own eigensolver
+    call on an operator too large for the scipy restatement above. Returns
+    (eigenvalues descending, eigenvectors n x k, number of matvecs)."""
+    import torch
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    u, d, a = T.scales(w)
+    r = u * a
+    w2 = w * w
+    count = [0]
+
+    def f(v):
+        count[0] += 1
+        x = torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64).reshape(-1)).to(r.device)
+        y = r * T.mm(w2 * T.rmm(r * x)) - x / d
+        return y.cpu().numpy()
+
+    n = T.n
+    np.random.seed(seed)
+    A = LinearOperator((n, n), matvec=f, dtype=np.float64)
+    evals, evecs = eigsh(A, k=k, v0=np.random.rand(n))
+    ix = evals.argsort()[::-1]
+    return evals[ix], evecs[:, ix], count[0]
+
+
+def eigen_parity(evals, U, ref_evals, ref_U, residuals, k: int) -> dict:
+    """Eigenpair agreement of a solver (evals, U; residual norms ||A u - lambda u||
+    against the FP64 operator) with the reference eigsh call (ref_evals, ref_U,
+    at least k + 1 pairs, descending), for the top k pairs by magnitude:
+      eig_rel_err      largest relative eigenvalue difference
+      gaps             lambda_j - lambda_{j+1} of the reference at j = 10 and k
+      separated        for every j <= k whose relative gap (lambda_j - lambda_{j+1})
+                       / |lambda_1| is at least 1e-3: sin of the largest angle
+                       between the top-j subspaces
+      sin_k            the same for all k pairs
+      dk_bound_k       its Davis-Kahan bound ||R||_F / delta, delta = smallest
+                       |lambda_i| (i <= k) minus |lambda_{k+1}| of the reference"""
+    ev = np.asarray(evals)[:k]
+    rv = np.asarray(ref_evals)
+    out = {"k": k, "eig_rel_err": float(np.max(np.abs(ev - rv[:k]) / np.abs(rv[:k])))}
+    gaps = {}
+    for j in (10, k):
+        if j < rv.size:
+            gaps[f"{j}-{j + 1}"] = float(rv[j - 1] - rv[j])
+    out["gaps"] = gaps
+    sep = {}
+    for j in range(1, k):
+        if (rv[j - 1] - rv[j]) / abs(rv[0]) >= 1e-3:
+            sep[j] = subspace_sin(U[:, :j], ref_U[:, :j])
+    out["separated"] = sep
+    out["sin_k"] = subspace_sin(U[:, :k], ref_U[:, :k])
+    if rv.size > k:
+        delta = float(np.min(np.abs(ev)) - abs(rv[k]))
+        out["dk_bound_k"] = float(np.linalg.norm(np.asarray(residuals)[:k]) / delta) if delta > 0 else float("inf")
+    return out

@@ -10,7 +10,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libsnapgpu.so"
 
-OK, ERR_ARG, ERR_HIP, ERR_DATA, ERR_CAPACITY = 0, 1, 2, 3, 4
+OK, ERR_ARG, ERR_HIP, ERR_DATA, ERR_CAPACITY, ERR_UNSORTED = 0, 1, 2, 3, 4, 5
 
 _p = C.c_void_p
 _i32, _i64, _u32, _u64 = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64
@@ -19,14 +19,14 @@ _i32, _i64, _u32, _u64 = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64
 SIGNATURES: dict[str, tuple] = {
     "snapgpu_last_error": (C.c_char_p, []),
     "snapgpu_abi_version": (C.c_int, []),
-    "snapgpu_copy_bytes": (C.c_int, [_p, _p, _i64, _p]),
-    "snapgpu_gather_probe": (C.c_int, [_p, _i64, _p, _i64, _p, _i64, _p]),
     "snapgpu_scan_workspace_bytes": (_i64, [_i64]),
     "snapgpu_scan_inclusive_i64": (C.c_int, [_p, _i64, _p, _i64, _p]),
     "snapgpu_tile_table_bytes": (_i64, [_i32]),
     "snapgpu_tile_table_pack": (C.c_int, [_p, _p, _i32, _u32, _p, C.POINTER(_u64)]),
     "snapgpu_tile_order_workspace_bytes": (_i64, [_i64]),
     "snapgpu_tile_order": (C.c_int, [_p, _i64, _p, _p, _i64, _p]),
+    "snapgpu_frag_sort_rows_workspace_bytes": (_i64, [_i64, _i64]),
+    "snapgpu_frag_sort_rows": (C.c_int, [_p, _i64, _i64, _p, _p, _p, _i64, _p]),
     "snapgpu_tile_count": (C.c_int, [_p, _p, _p, _i64, _p, _i32, _u32, _i32, _u32, _u32, _p, _p, _p,
                                      C.POINTER(_i32), _p]),
     "snapgpu_tile_fill": (C.c_int, [_p, _p, _p, _i64, _p, _i32, _u32, _i32, _u32, _u32, _i32, _p,
@@ -35,6 +35,7 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_csr_col_sum_rows_u32": (C.c_int, [_p, _i64, _p, _i32, _p, _i64, _p, _p, _p]),
     "snapgpu_synth_counts": (C.c_int, [_u64, _i64, _i64, C.c_double, C.c_double, _i64, _i64, _p,
                                        _p]),
+    "snapgpu_synth_cell_types": (C.c_int, [_u64, _i64, _i64, _i32, _p, _p]),
     "snapgpu_synth_workspace_bytes": (_i64, [_i64]),
     "snapgpu_synth_fragments": (C.c_int, [_u64, _i64, _i64, _p, _i64, _p, _i32, _p, _i32, _p,
                                           _i32, C.c_double, C.c_double, _p, _p, _p, _i64, _p]),
@@ -65,6 +66,15 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_spectral_apply_panel_workspace_bytes": (_i64, [_i64, _i64, _i64, _i64]),
     "snapgpu_spectral_apply_panel": (C.c_int, [_p, _p, _p, _p, _p, _p, _i64, _i32, _p, _i64, _p,
                                                _i64, _p]),
+    "snapgpu_block_panel_rows": (_i32, []),
+    "snapgpu_block_plan_workspace_bytes": (_i64, [_i64]),
+    "snapgpu_block_plan": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _i32, _i64, _p, _i64, C.POINTER(_i64),
+                                     C.POINTER(_i64), C.POINTER(_i32), _p]),
+    "snapgpu_block_build_workspace_bytes": (_i64, [_i64]),
+    "snapgpu_block_build": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _i32, _p, _i64, _i64, _i64, _p, _i64, _p, _p,
+                                      _p, _p, _p, _p, _p, _p]),
+    "snapgpu_spectral_apply_block_workspace_bytes": (_i64, [_i64, _i64]),
+    "snapgpu_spectral_apply_block": (C.c_int, [_p, _p, _p, _p, _p, _p, _i64, _i32, _p, _i64, _p, _i64, _p]),
     "snapgpu_gemm_tn_workspace_bytes": (_i64, [_i64, _i32, _i32]),
     "snapgpu_gemm_tn_f64": (C.c_int, [_p, _i64, _p, _i64, _i64, _i32, _i32, _p, _p, _i64, _p]),
     "snapgpu_gemm_nn_f64": (C.c_int, [_p, _i64, _p, _i64, _i32, _i32, C.c_double, C.c_double,
@@ -108,6 +118,13 @@ class PanelLayout(C.Structure):
                 ("chunk_panel", _p), ("n_chunks", _i64), ("row_pair_ptr", _p),
                 ("row_pair_ids", _p), ("n_pairs", _i64), ("n_rows", _i64), ("dense_rows", _i64),
                 ("packed", _i32), ("value_type", _i32)]
+
+
+class BlockLayout(C.Structure):
+    """snapgpu_block_layout (include/snapgpu.h): device pointers and sizes."""
+    _fields_ = [("ent", _p), ("val", _p), ("tile_start", _p), ("row0", _p), ("out_row", _p), ("vslot", _p),
+                ("nslots", _p), ("n_blocks", _i64), ("n_panels", _i64), ("n_rows", _i64), ("dense_rows", _i64),
+                ("n_entries", _i64), ("packed", _i32), ("value_type", _i32)]
 
 
 class PeakIndex(C.Structure):

@@ -1,4 +1,5 @@
-"""Builds libsnapgpu.so (all HIP sources under csrc/) for gfx950, in-tree.
+"""Builds libsnapgpu.so (the HIP sources in csrc/) and the bench-only
+libsnapgpu_probe.so (csrc/probe/) for gfx950, in-tree.
 
 Usage: python -m snapatac2_amd.build
 """
@@ -30,20 +31,29 @@ HIPCC_FLAGS = [
 ]
 
 
+PROBE_LIB = OUT_DIR / "libsnapgpu_probe.so"
+
+
 def sources() -> list[Path]:
     return sorted(CSRC.glob("*.hip"))
 
 
-def build(verbose: bool = True) -> Path:
+def _compile(srcs, out: Path, includes, verbose: bool) -> None:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    OUT_DIR.mkdir(exist_ok=True)
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc, *HIPCC_FLAGS, f"-I{ROOT / 'include'}", f"-I{CSRC}",
-           *map(str, sources()), "-o", str(tmp)]
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [hipcc, *HIPCC_FLAGS, *(f"-I{d}" for d in includes), *map(str, srcs), "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = True) -> Path:
+    """libsnapgpu.so (the product) and libsnapgpu_probe.so (bench.py's
+    measurement aids, csrc/probe/), both for gfx950."""
+    OUT_DIR.mkdir(exist_ok=True)
+    _compile(sources(), LIB, [ROOT / "include", CSRC], verbose)
+    _compile(sorted((CSRC / "probe").glob("*.hip")), PROBE_LIB, [CSRC / "probe"], verbose)
     return LIB
 
 

@@ -32,6 +32,7 @@
 //     the largest norm in the data. Rows that fail are rescanned exactly by the
 //     fallback kernel.
 //  4. output: k neighbours per row sorted by index, distance = sqrt(d^2).
+#include <algorithm>
 #include <cfloat>
 #include <vector>
 
@@ -953,6 +954,163 @@ __global__ void __launch_bounds__(256) finalize_multi_kernel(FinParams P, SegPar
   }
 }
 
+// ---------------------------------------------------------------------------
+// 5. wide path (d > 30 or k > 100, beyond the candidate lists above): exact
+// FP64 distances from a block of queries to every point, in the oracle's
+// arithmetic (dimension order, no fused multiply-add), keyed by their bit
+// patterns (non-negative doubles order like their bits; the query itself gets
+// the +inf key), then one stable radix sort per query: equal distances keep
+// index order, so the first k entries are the exact neighbours with ties broken
+// by index. A second stable sort puts each row's k neighbours in index order.
+// O(n^2 d) arithmetic; the points are read transposed, so a block's 256 lanes
+// load consecutive points of one dimension, and each lane serves kWideQ queries.
+// ---------------------------------------------------------------------------
+constexpr int kWideQ = 8;                     // queries per block
+constexpr int64_t kWideItems = 1ll << 27;     // (query, point) pairs per sort
+
+__global__ void transpose_kernel(const double* __restrict__ X, int64_t n, int d, int64_t ld,
+                                 double* __restrict__ Xt) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * d) return;
+  const int64_t c = t / n, p = t - c * n;
+  Xt[t] = X[p * ld + c];
+}
+
+__global__ void __launch_bounds__(256) wide_dist_kernel(const double* __restrict__ X, const double* __restrict__ Xt,
+                                                        int64_t n, int d, int64_t ld, int64_t q0, int64_t nq,
+                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t qb = (int64_t)blockIdx.y * kWideQ;
+  if (p >= n) return;
+  double acc[kWideQ];
+  const double* xq[kWideQ];
+#pragma unroll
+  for (int j = 0; j < kWideQ; ++j) {
+    acc[j] = 0.0;
+    xq[j] = X + (q0 + min(qb + j, nq - 1)) * ld;  // rows past the block repeat its last query
+  }
+  for (int c = 0; c < d; ++c) {
+    const double x = Xt[(int64_t)c * n + p];
+#pragma unroll
+    for (int j = 0; j < kWideQ; ++j) {
+      const double t = xq[j][c] - x;
+      acc[j] = acc[j] + t * t;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kWideQ; ++j) {
+    const int64_t qi = qb + j;
+    if (qi >= nq) break;
+    const int64_t o = qi * n + p;
+    keys[o] = (q0 + qi == p) ? 0x7FF0000000000000ull : (uint64_t)__double_as_longlong(acc[j]);
+    vals[o] = (uint32_t)p;
+  }
+}
+
+__global__ void iota_offsets_kernel(int64_t* off, int64_t count, int64_t step) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= count) off[i] = i * step;
+}
+
+// the k nearest of each sorted row: (neighbour index, d^2 bits) for the second sort
+__global__ void wide_take_kernel(const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ svals, int64_t n,
+                                 int64_t nq, int k, uint32_t* __restrict__ idx, uint64_t* __restrict__ d2) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq * k) return;
+  const int64_t qi = t / k, r = t - qi * k;
+  idx[t] = svals[qi * n + r];
+  d2[t] = skeys[qi * n + r];
+}
+
+__global__ void wide_write_kernel(const uint32_t* __restrict__ idx, const uint64_t* __restrict__ d2, int64_t q0,
+                                  int64_t nq, int k, int32_t* __restrict__ nbr, double* __restrict__ dist) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq * k) return;
+  nbr[q0 * k + t] = (int32_t)idx[t];
+  dist[q0 * k + t] = sqrt(__longlong_as_double((long long)d2[t]));
+}
+
+struct WideWs {
+  double* xt;
+  uint64_t *k0, *k1, *tk0, *tk1;
+  uint32_t *v0, *v1, *ti0, *ti1;
+  int64_t *off, *off_k;
+  void* tmp;
+  size_t tmp_bytes;
+  int64_t block_q;
+};
+
+static int64_t wide_block_queries(int64_t n) {
+  return std::max<int64_t>(1, std::min<int64_t>(kWideItems / n, n));
+}
+
+static int64_t wide_layout(int64_t n, int d, int k, WideWs* w, char* base) {
+  int64_t off = 0;
+  auto take = [&](int64_t bytes) -> char* {
+    char* p = base ? base + off : nullptr;
+    off += (bytes + 255) / 256 * 256;
+    return p;
+  };
+  const int64_t bq = wide_block_queries(n);
+  const int64_t items = bq * n, kitems = bq * k;
+  WideWs t;
+  t.block_q = bq;
+  t.xt = (double*)take(n * d * 8);
+  t.k0 = (uint64_t*)take(items * 8);
+  t.k1 = (uint64_t*)take(items * 8);
+  t.v0 = (uint32_t*)take(items * 4);
+  t.v1 = (uint32_t*)take(items * 4);
+  t.tk0 = (uint64_t*)take(kitems * 8);
+  t.tk1 = (uint64_t*)take(kitems * 8);
+  t.ti0 = (uint32_t*)take(kitems * 4);
+  t.ti1 = (uint32_t*)take(kitems * 4);
+  t.off = (int64_t*)take((bq + 1) * 8);
+  t.off_k = (int64_t*)take((bq + 1) * 8);
+  size_t a = 0, b = 0;
+  (void)hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, a, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                    (uint32_t*)nullptr, (uint32_t*)nullptr, (int)items, (int)bq,
+                                                    (int64_t*)nullptr, (int64_t*)nullptr, 0, 63);
+  (void)hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, b, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                    (uint64_t*)nullptr, (uint64_t*)nullptr, (int)kitems, (int)bq,
+                                                    (int64_t*)nullptr, (int64_t*)nullptr, 0, 32);
+  t.tmp_bytes = std::max(a, b);
+  t.tmp = take((int64_t)t.tmp_bytes + 256);
+  if (w) *w = t;
+  return off;
+}
+
+static bool wide_needed(int d, int k) { return d > kMaxData || k > kMaxK; }
+
+static int wide_knn(const double* X, int64_t n, int d, int64_t ld, int k, int32_t* nbr, double* dist, char* base,
+                    hipStream_t st) {
+  WideWs w;
+  wide_layout(n, d, k, &w, base);
+  transpose_kernel<<<(unsigned)ceil_div(n * d, 256), 256, 0, st>>>(X, n, d, ld, w.xt);
+  SG_LAUNCH_CHECK();
+  for (int64_t q0 = 0; q0 < n; q0 += w.block_q) {
+    const int64_t nq = std::min<int64_t>(w.block_q, n - q0);
+    const dim3 grid((unsigned)ceil_div(n, 256), (unsigned)ceil_div(nq, kWideQ));
+    wide_dist_kernel<<<grid, 256, 0, st>>>(X, w.xt, n, d, ld, q0, nq, w.k0, w.v0);
+    iota_offsets_kernel<<<(unsigned)ceil_div(nq + 1, 256), 256, 0, st>>>(w.off, nq, n);
+    iota_offsets_kernel<<<(unsigned)ceil_div(nq + 1, 256), 256, 0, st>>>(w.off_k, nq, k);
+    SG_LAUNCH_CHECK();
+    size_t tb = w.tmp_bytes;
+    // keys below 2^63 (non-negative doubles and the +inf key): 63 bits
+    SG_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(w.tmp, tb, w.k0, w.k1, w.v0, w.v1, (int)(nq * n), (int)nq,
+                                                       w.off, w.off + 1, 0, 63, st));
+    wide_take_kernel<<<(unsigned)ceil_div(nq * k, 256), 256, 0, st>>>(w.k1, w.v1, n, nq, k, w.ti0, w.tk0);
+    SG_LAUNCH_CHECK();
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) < n) ++bits;
+    tb = w.tmp_bytes;
+    SG_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(w.tmp, tb, w.ti0, w.ti1, w.tk0, w.tk1, (int)(nq * k),
+                                                       (int)nq, w.off_k, w.off_k + 1, 0, bits, st));
+    wide_write_kernel<<<(unsigned)ceil_div(nq * k, 256), 256, 0, st>>>(w.ti1, w.tk1, q0, nq, k, nbr, dist);
+    SG_LAUNCH_CHECK();
+  }
+  return SNAPGPU_OK;
+}
+
 struct Workspace {
   float *xf, *qf, *qnorm, *tau;
   int32_t *cand, *fail_rows, *fail_rows2;
@@ -1022,9 +1180,8 @@ using namespace snapgpu::knn;
 extern "C" {
 
 int64_t snapgpu_knn_workspace_bytes(int64_t n, int32_t d, int32_t k) {
-  (void)d;
-  (void)k;
   if (n < 0) return 0;
+  if (wide_needed(d, k)) return wide_layout(n, d, std::max(k, 1), nullptr, nullptr) + 256;
   return layout(n, nullptr, nullptr) + 256;
 }
 
@@ -1035,14 +1192,22 @@ int snapgpu_knn(const double* d_X, int64_t n, int32_t d, int64_t ld, int32_t k, 
   clear_error();
   SG_ARG(n >= 2, "need at least 2 points");
   SG_ARG(n < INT32_MAX, "too many points");
-  SG_ARG(d >= 1 && d <= kMaxData, "dimension must be in [1, 30]");
+  SG_ARG(d >= 1, "dimension must be >= 1");
   SG_ARG(ld >= d, "ld < d");
-  SG_ARG(k >= 1 && k <= kMaxK && k <= n - 1, "k must be in [1, min(100, n-1)]");
+  SG_ARG(k >= 1 && k <= n - 1, "k must be in [1, n-1]");
   SG_ARG(mode == kModeFP32 || mode == kModeBF16, "mode must be 0 (fp32) or 1 (bf16)");
   SG_ARG(d_X && d_nbr && d_dist && d_workspace, "null pointer");
   SG_ARG(workspace_bytes >= snapgpu_knn_workspace_bytes(n, d, k), "workspace too small");
   hipStream_t st = as_stream(stream);
   char* base = (char*)(((uintptr_t)d_workspace + 255) & ~(uintptr_t)255);
+  if (wide_needed(d, k)) {
+    const int rc = wide_knn(d_X, n, d, ld, k, d_nbr, d_dist, base, st);
+    if (rc) return rc;
+    if (n_fallback) *n_fallback = 0;
+    if (n_exact) *n_exact = n;
+    if (n_chunk_visits) *n_chunk_visits = 0;
+    return SNAPGPU_OK;
+  }
   Workspace w;
   layout(n, &w, base);
   const int64_t n_pad = ((n + 15) / 16) * 16;

@@ -118,6 +118,14 @@ __global__ void unpack_kernel(const uint64_t* keys, int64_t n, uint32_t* pos, ui
   }
 }
 
+// Cell type of each cell: the same draw gen_kernel makes for the cell.
+__global__ void types_kernel(uint64_t seed, int64_t first_cell, int64_t n_cells, int32_t n_types,
+                             int32_t* types) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_cells) return;
+  types[i] = (int32_t)(unif(rng(seed, (uint64_t)(first_cell + i), ~0ull, 3)) * n_types);
+}
+
 }  // namespace synth
 }  // namespace snapgpu
 
@@ -133,6 +141,17 @@ int snapgpu_synth_counts(uint64_t seed, int64_t first_cell, int64_t n_cells, dou
   if (n_cells == 0) return SNAPGPU_OK;
   counts_kernel<<<(unsigned)ceil_div(n_cells, 256), 256, 0, as_stream(stream)>>>(
       seed, first_cell, n_cells, mu, sigma, min_frags, max_frags, d_counts);
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
+int snapgpu_synth_cell_types(uint64_t seed, int64_t first_cell, int64_t n_cells, int32_t n_types,
+                             int32_t* d_types, void* stream) {
+  clear_error();
+  SG_ARG(n_cells >= 0 && first_cell >= 0 && n_types >= 1, "bad cell range or type count");
+  if (n_cells == 0) return SNAPGPU_OK;
+  types_kernel<<<(unsigned)ceil_div(n_cells, 256), 256, 0, as_stream(stream)>>>(seed, first_cell, n_cells,
+                                                                              n_types, d_types);
   SG_LAUNCH_CHECK();
   return SNAPGPU_OK;
 }

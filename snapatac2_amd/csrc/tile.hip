@@ -23,6 +23,7 @@
 // wave), run-length encoded with ballots, and merged with the pending list by
 // rank (binary search in LDS). The same code runs twice: pass 1 only counts, pass 2 writes.
 #include <hipcub/hipcub.hpp>
+#include <algorithm>
 
 #include "common.h"
 
@@ -65,6 +66,7 @@ enum : int32_t {
   kErrZeroLength = 3,
   kErrFillMismatch = 5,
   kErrSpan = 6,
+  kErrUnsorted = 7,
 };
 
 // "fragment" counting expands every covered bin; longer spans are rejected.
@@ -328,6 +330,7 @@ __device__ void process_cell(const Params& p, const ChromRec* tab, WaveLds<CAP>*
   // the next tile's positions and lengths are loaded one tile ahead (clamped
   // to the cell's last fragment, so every load is in bounds and unconditional)
   uint32_t ps_n = 0u, ln_n = 1u;
+  uint32_t prev_last = 0u;  // position of the previous tile's last fragment
   if (f0 < f1) {
     const int64_t fc = min(f0 + lane, f1 - 1);
     ps_n = p.pos[fc];
@@ -342,6 +345,18 @@ __device__ void process_cell(const Params& p, const ChromRec* tab, WaveLds<CAP>*
       const int64_t fc = min(f + 64, f1 - 1);
       ps_n = p.pos[fc];
       if (paired) ln_n = p.len[fc];
+    }
+    // Positions must not decrease along the row (the final-bin bound T below
+    // relies on it); a row out of order is reported, and the caller sorts it.
+    {
+      uint32_t prev = (uint32_t)__shfl_up((int)ps, 1, 64);
+      if (lane == 0) prev = prev_last;
+      if (ballot64(has && ps < prev) != 0) {
+        if (lane == 0) report(p, kErrUnsorted, cell, 0);
+        s.np = 0;
+        break;
+      }
+      prev_last = (uint32_t)__shfl((int)ps, (int)min<int64_t>(63, f1 - 1 - t0), 64);
     }
     bool kept = false;
     uint32_t b1 = kInf, b2 = kInf;
@@ -527,6 +542,10 @@ static int check_status(const int32_t* d_status, hipStream_t st, bool* overflow)
         msg = "a fragment covers " + std::to_string(h[3]) +
               " bins (limit 2^20 for counting_strategy='fragment'); use a larger bin_size";
         break;
+      case kErrUnsorted:
+        set_error("the fragments of cell " + std::to_string(cell) +
+                  " are not sorted by position; sort the rows (snapgpu_frag_sort_rows) and call again");
+        return SNAPGPU_ERR_UNSORTED;
       case kErrFillMismatch:
         msg = "tile fill produced a row length different from the count pass (cell " +
               std::to_string(cell) + ")";
@@ -549,6 +568,22 @@ __global__ void order_keys_kernel(const uint64_t* cell_ptr, int64_t n, uint32_t*
   const uint64_t len = cell_ptr[i + 1] - cell_ptr[i];
   keys[i] = ~(uint32_t)min(len, (uint64_t)0xFFFFFFFFu);  // ascending keys = descending sizes
   ids[i] = (uint32_t)i;
+}
+
+// (position, value) as one 64-bit key, so a key sort orders a row by position
+// and then value; single-end rows (no values) sort by position alone.
+__global__ void pack_sort_keys_kernel(const uint32_t* pos, const uint32_t* len, int64_t n, uint64_t* keys) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = ((uint64_t)pos[i] << 32) | (len ? len[i] : 0u);
+}
+
+__global__ void unpack_sort_keys_kernel(const uint64_t* keys, int64_t n, uint32_t* pos, uint32_t* len) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = keys[i];
+  pos[i] = (uint32_t)(k >> 32);
+  if (len) len[i] = (uint32_t)k;
 }
 
 static int make_params(Params& p, const uint64_t* cell_ptr, const uint32_t* pos, const uint32_t* len,
@@ -643,6 +678,41 @@ int snapgpu_tile_order(const uint64_t* d_cell_ptr, int64_t n_cells, uint32_t* d_
   SG_LAUNCH_CHECK();
   SG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys2, ids, d_order, (int)n_cells, 0, 32,
                                             st));
+  return SNAPGPU_OK;
+}
+
+int64_t snapgpu_frag_sort_rows_workspace_bytes(int64_t n_cells, int64_t n_fragments) {
+  const int64_t f = std::max<int64_t>(n_fragments, 1);
+  size_t tmp = 0;
+  (void)hipcub::DeviceSegmentedSort::SortKeys(nullptr, tmp, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                              (int)std::min<int64_t>(f, INT32_MAX),
+                                              (int)std::min<int64_t>(std::max<int64_t>(n_cells, 1), INT32_MAX),
+                                              (const uint64_t*)nullptr, (const uint64_t*)nullptr);
+  return 2 * ((f * 8 + 255) / 256 * 256) + (int64_t)tmp + 256;
+}
+
+int snapgpu_frag_sort_rows(const uint64_t* d_cell_ptr, int64_t n_cells, int64_t n_fragments, uint32_t* d_pos,
+                           uint32_t* d_len, void* d_ws, int64_t ws_bytes, void* stream) {
+  clear_error();
+  SG_ARG(n_cells >= 0 && n_cells < INT32_MAX, "bad cell count");
+  SG_ARG(n_fragments >= 0 && n_fragments < INT32_MAX, "n_fragments must be below 2^31 (sort chunks of rows)");
+  if (n_cells == 0 || n_fragments == 0) return SNAPGPU_OK;
+  SG_ARG(d_cell_ptr && d_pos && d_ws, "null pointer");
+  SG_ARG(ws_bytes >= snapgpu_frag_sort_rows_workspace_bytes(n_cells, n_fragments), "workspace too small");
+  hipStream_t st = as_stream(stream);
+  char* q = (char*)(((uintptr_t)d_ws + 255) & ~(uintptr_t)255);
+  const int64_t arr = (n_fragments * 8 + 255) / 256 * 256;
+  uint64_t* keys = (uint64_t*)q;
+  uint64_t* keys2 = (uint64_t*)(q + arr);
+  void* tmp = q + 2 * arr;
+  size_t tmp_bytes = (size_t)(ws_bytes - (2 * arr + (q - (char*)d_ws)));
+  const unsigned g = (unsigned)ceil_div(n_fragments, 256);
+  pack_sort_keys_kernel<<<g, 256, 0, st>>>(d_pos, d_len, n_fragments, keys);
+  SG_LAUNCH_CHECK();
+  SG_HIP(hipcub::DeviceSegmentedSort::SortKeys(tmp, tmp_bytes, keys, keys2, (int)n_fragments, (int)n_cells,
+                                               d_cell_ptr, d_cell_ptr + 1, st));
+  unpack_sort_keys_kernel<<<g, 256, 0, st>>>(keys2, n_fragments, d_pos, d_len);
+  SG_LAUNCH_CHECK();
   return SNAPGPU_OK;
 }
 

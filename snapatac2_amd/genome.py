@@ -85,6 +85,15 @@ class BinIndex:
         c, s, e = self.region(int(i))
         return f"{c}:{s}-{e}"
 
+    def ranges(self) -> tuple[list[str], np.ndarray, np.ndarray, np.ndarray]:
+        """(chromosome names, chromosome code, start, end) of every bin."""
+        nb = (self.lengths + self.step - 1) // self.step
+        code = np.repeat(np.arange(len(self.chroms), dtype=np.int32), nb)
+        first = np.concatenate([[0], self.bin_accum[:-1]]).astype(np.int64) if len(nb) else np.zeros(0, np.int64)
+        start = (np.arange(len(self), dtype=np.int64) - np.repeat(first, nb)) * self.step
+        end = np.minimum(start + self.step, np.repeat(self.lengths, nb))
+        return list(self.chroms), code, start, end
+
     def to_numpy(self) -> np.ndarray:
         out = []
         for c, L in zip(self.chroms, self.lengths):

@@ -20,6 +20,7 @@ from .. import _lib
 from .._device import DeviceCSR, DeviceFragments
 from ..anndata import AnnData
 from ..genome import BinIndex, read_chrom_sizes
+from ._regions import intersect_bed
 
 FRAGMENT_SINGLE = "fragment_single"  # snapatac2-core/src/feature_count/mod.rs:20
 FRAGMENT_PAIRED = "fragment_paired"  # mod.rs:23
@@ -28,6 +29,40 @@ _STRATEGY = {"insertion": 0, "fragment": 1, "paired-insertion": 2}
 _SINGLE = 3
 _VALUE_TYPES = ("target", "total", "fraction")        # preprocessing.rs:347-354
 _SUMMARY_TYPES = ("sum", "mean", "count")             # preprocessing.rs:356-363
+
+
+def _sorted_rows(frags: DeviceFragments) -> DeviceFragments:
+    """A copy of `frags` with every row sorted by (position, value)
+    (snapgpu_frag_sort_rows), for the tile passes, which need rows in position
+    order; the reference's per-row BTreeMap (data_iter.rs:362-411) gives the same
+    matrix for any order."""
+    torch = _lib.require_gpu()
+    L = _lib.lib()
+    pos = frags.pos.clone()
+    vals = frags.values.clone()
+    n, F = frags.shape[0], frags.n_fragments
+    wsb = int(L.snapgpu_frag_sort_rows_workspace_bytes(n, F))
+    ws = torch.empty(wsb, dtype=torch.uint8, device=frags.device)
+    _lib.check(L.snapgpu_frag_sort_rows(frags.cell_ptr.data_ptr(), n, F, pos.data_ptr(), vals.data_ptr(),
+                                        ws.data_ptr(), wsb, _lib.stream_ptr()))
+    return DeviceFragments(frags.cell_ptr, pos, vals, frags.n_cols, frags.paired)
+
+
+def _tile_count(L, frags, table, n_chrom, bin_size, strategy, min_len, max_len, row_nnz_ptr, order,
+                status, level, stream) -> DeviceFragments:
+    """snapgpu_tile_count; a row out of position order is sorted (on a copy) and
+    the pass repeated. Returns the fragments the fill pass must read."""
+    for attempt in range(2):
+        rc = L.snapgpu_tile_count(frags.cell_ptr.data_ptr(), _lib.ptr(frags.pos),
+                                  _lib.ptr(frags.values) if frags.paired else 0, frags.shape[0],
+                                  table.data_ptr(), n_chrom, int(bin_size), strategy, min_len, max_len,
+                                  row_nnz_ptr, _lib.ptr(order), status.data_ptr(), C.byref(level), stream)
+        if rc == _lib.ERR_UNSORTED and attempt == 0:
+            frags = _sorted_rows(frags)
+            level.value = 0
+            continue
+        _lib.check(rc)
+        return frags
 
 
 def _as_device_fragments(m, paired: bool) -> DeviceFragments:
@@ -90,10 +125,9 @@ def make_tile_matrix(
     level = C.c_int32(0)
     row_nnz_ptr = indptr.data_ptr() + 8
     order = _cell_order(torch, L, cell_ptr, n, stream)
-    _lib.check(L.snapgpu_tile_count(cell_ptr.data_ptr(), _lib.ptr(frags.pos), lens_ptr, n,
-                                    table.data_ptr(), n_chrom, int(bin_size), strategy,
-                                    min_len, max_len, row_nnz_ptr, _lib.ptr(order), status.data_ptr(),
-                                    C.byref(level), stream))
+    frags = _tile_count(L, frags, table, n_chrom, bin_size, strategy, min_len, max_len, row_nnz_ptr, order,
+                        status, level, stream)
+    lens_ptr = _lib.ptr(frags.values) if frags.paired else 0
     ws_bytes = int(L.snapgpu_scan_workspace_bytes(n))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     _lib.check(L.snapgpu_scan_inclusive_i64(row_nnz_ptr, n, ws.data_ptr(), ws_bytes, stream))
@@ -185,15 +219,14 @@ def make_tile_matrix_chunked(chunks, chrom_sizes: dict[str, int], bin_size: int 
 
     orders = [None] * len(chunks)
 
+    frs = [f for _, f in chunks]  # a chunk with a row out of order is replaced by a sorted copy
+
     def count(ci, status, stream):
         c0, fr = chunks[ci]
         level = C.c_int32(0)
         orders[ci] = _cell_order(torch, L, fr.cell_ptr, fr.shape[0], stream)
-        _lib.check(L.snapgpu_tile_count(fr.cell_ptr.data_ptr(), fr.pos.data_ptr(),
-                                        _lib.ptr(fr.values) if fr.paired else 0, fr.shape[0],
-                                        table.data_ptr(), n_chrom, int(bin_size), strategy,
-                                        min_len, max_len, indptr.data_ptr() + 8 * (c0 + 1),
-                                        _lib.ptr(orders[ci]), status.data_ptr(), C.byref(level), stream))
+        frs[ci] = _tile_count(L, fr, table, n_chrom, bin_size, strategy, min_len, max_len,
+                              indptr.data_ptr() + 8 * (c0 + 1), orders[ci], status, level, stream)
         levels[ci] = level.value
 
     run(count)
@@ -209,7 +242,8 @@ def make_tile_matrix_chunked(chunks, chrom_sizes: dict[str, int], bin_size: int 
         st_.wait_stream(main)
 
     def fill(ci, status, stream):
-        c0, fr = chunks[ci]
+        c0 = chunks[ci][0]
+        fr = frs[ci]
         _lib.check(L.snapgpu_tile_fill(fr.cell_ptr.data_ptr(), fr.pos.data_ptr(),
                                        _lib.ptr(fr.values) if fr.paired else 0, fr.shape[0],
                                        table.data_ptr(), n_chrom, int(bin_size), strategy,
@@ -355,15 +389,16 @@ def select_features(
     """Select the most accessible features (drop-in for snapatac2.pp.select_features).
 
     Column sums run on the GPU; the selection itself is the reference's numpy
-    code. Whitelist/blacklist BED files and max_iter > 1 (iterative clustering)
-    are not supported.
+    code. Whitelist and blacklist BED files are tested against var_names with
+    the peak-matrix overlap kernel (intersect_bed, utils.rs:320-336), applied as
+    in _basic.py:1062-1087: blacklisted features leave the selection, then
+    non-zero whitelisted features join it. max_iter > 1 (iterative clustering,
+    which needs Leiden) is not supported.
     """
     if isinstance(adata, list):
         res = [select_features(a, n_features, filter_lower_quantile, filter_upper_quantile,
                                whitelist, blacklist, max_iter, inplace, verbose=False) for a in adata]
         return None if inplace else res
-    if whitelist is not None or blacklist is not None:
-        raise NotImplementedError("whitelist/blacklist BED filtering is not supported")
     if max_iter > 1:
         raise NotImplementedError("iterative feature selection (max_iter > 1) is not supported")
     count = _column_sums(adata.X)
@@ -371,8 +406,14 @@ def select_features(
         adata.var["count"] = count
     selected = _find_most_accessible_features(count, filter_lower_quantile,
                                               filter_upper_quantile, n_features)
+    if blacklist is not None:
+        black = intersect_bed(adata.var_names, str(blacklist))
+        selected = selected[np.logical_not(black[selected])]
     result = np.zeros(adata.shape[1], dtype=bool)
     result[selected] = True
+    if whitelist is not None:
+        white = intersect_bed(adata.var_names, str(whitelist))
+        result |= white & (count != 0)
     if verbose:
         logging.info(f"Selected {result.sum()} features.")
     if inplace:

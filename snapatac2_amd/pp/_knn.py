@@ -19,9 +19,12 @@ _MODES = {"fp32": 0, "bf16": 1}
 
 def nearest_neighbour_graph(data, k: int, precision: str = "bf16", force_fallback: bool = False,
                             return_stats: bool = False, prune: bool = True):
-    """Exact kNN graph of the rows of `data` (n x d, d <= 30): scipy CSR n x n,
-    float64 Euclidean distances, k entries per row sorted by neighbour index,
-    self excluded by index. `data` may be a numpy array or a CUDA torch tensor.
+    """Exact kNN graph of the rows of `data` (n x d): scipy CSR n x n, float64
+    Euclidean distances, k entries per row sorted by neighbour index, self
+    excluded by index. `data` may be a numpy array or a CUDA torch tensor.
+    For d <= 30 and k <= 100, `precision` picks the candidate arithmetic (bf16
+    MFMA or fp32) ahead of the exact FP64 rescoring; wider rows or more
+    neighbours take an exact FP64 all-pairs scan (both precisions).
     prune=False makes the bf16 path visit every database chunk (same result)."""
     torch = _lib.require_gpu()
     L = _lib.lib()

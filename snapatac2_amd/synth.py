@@ -54,6 +54,17 @@ def fragment_counts(n_cells: int, seed: int = 0, first_cell: int = 0, device=Non
     return out
 
 
+def cell_types(n_cells: int, seed: int = 0, first_cell: int = 0, n_types: int = 10, device=None):
+    """The generator's cell type of each cell (int32 device tensor): the type
+    whose peak activity the cell's peak fragments are drawn from."""
+    torch = _lib.require_gpu()
+    dev = device or torch.device("cuda")
+    out = torch.empty(n_cells, dtype=torch.int32, device=dev)
+    _lib.check(_lib.lib().snapgpu_synth_cell_types(seed, first_cell, n_cells, n_types, out.data_ptr(),
+                                                   _lib.stream_ptr()))
+    return out
+
+
 def _scan_to_indptr(counts):
     torch = _lib.require_gpu()
     L = _lib.lib()

@@ -93,22 +93,68 @@ def spectral_gap(op: SpectralOperator, n_components: int = 30, seed: int = 0, v0
     """(gap, evals, U): gap = |lambda_k| - |lambda_{k+1}| between the n_components-th
     and the next eigenvalue in the largest-magnitude order used by spectral().
     v0: eigenvectors of a nearby operator to start the eigensolver from."""
-    evals, U, _ = eigsh_lm(op, n_components + 1, seed=seed, v0=v0)
+    gap, evals, U, _ = _gap_stats(op, n_components, seed, v0)
+    return gap, evals, U
+
+
+def _gap_stats(op: SpectralOperator, n_components: int, seed: int, v0=None):
+    evals, U, st = eigsh_lm(op, n_components + 1, seed=seed, v0=v0)
     k = n_components
-    return float(abs(evals[k - 1]) - abs(evals[k])), evals, U
+    return float(abs(evals[k - 1]) - abs(evals[k])), evals, U, st
+
+
+def _checked_gap(op: SpectralOperator, evals, U, st, k: int):
+    """(gap, uncertainty) of |lambda_k| - |lambda_{k+1}| for an acceptance test.
+    With integer counts the two eigenvalues are re-estimated as FP64 Rayleigh
+    quotients of their Lanczos vectors, so the FP32 products' rounding (about
+    1e-7 of |lambda_1|, the size of the gap at a million cells) drops out; what
+    remains is the Rayleigh quotient's error, at most residual^2 / gap per
+    eigenvalue, plus FP64 summation rounding."""
+    res = float(st.get("max_residual", 0.0)) if isinstance(st, dict) else 0.0
+    if op.value_type != 0:
+        gap = float(abs(evals[k - 1]) - abs(evals[k]))
+        return gap, 2.0 * res + 1e-6 * abs(float(evals[0]))
+    _, (l0, l1) = _pair_f64(op, U, evals, k)
+    gap = abs(l0) - abs(l1)
+    return gap, 2.0 * res * res / max(abs(gap), 1e-300) + 1e-12 * abs(float(evals[0]))
 
 
 def spectral_gap_grad(op: SpectralOperator, n_components: int = 30, seed: int = 0, v0=None):
     """(gap, dgap/dw, evals, U) by Hellmann-Feynman: d lambda_i = u_i^T dA u_i, i.e. the adjoint
     with V = [u_k, u_{k+1}] and G = [sign(lambda_k) u_k, -sign(lambda_{k+1}) u_{k+1}]."""
-    torch = op.torch
-    gap, evals, U = spectral_gap(op, n_components, seed, v0)
-    k = n_components
+    gap, g, evals, U, _ = _gap_grad_stats(op, n_components, seed, v0)
+    return gap, g, evals, U
+
+
+def _pair_f64(op: SpectralOperator, U, evals, k: int):
+    """The k-th and (k+1)-th Lanczos pairs refined in FP64: Rayleigh-Ritz of the
+    FP64 operator on their 2-dimensional span. When the two eigenvalues are
+    close, the FP32 products' rounding E mixes the two vectors by an angle of
+    about |E| / gap (0.1 rad at a million cells), which the Hellmann-Feynman
+    gradient of each eigenvalue would inherit; the 2 x 2 rotation removes that
+    mixing. Returns (V, (lambda_k, lambda_{k+1})); V unchanged (and the Lanczos
+    eigenvalues) for float-valued matrices."""
     V = U[:, k - 1:k + 1].contiguous()
-    sg = torch.tensor([np.sign(evals[k - 1]) or 1.0, -(np.sign(evals[k]) or 1.0)],
-                      dtype=torch.float64, device=V.device)
+    AV = op.apply_f64(V)
+    if AV is None:
+        return V, (float(evals[k - 1]), float(evals[k]))
+    H = (V.T @ AV).cpu().numpy()
+    H = 0.5 * (H + H.T)
+    lam, Q = np.linalg.eigh(H)
+    order = np.argsort(-np.abs(lam), kind="stable")
+    lam, Q = lam[order], Q[:, order]
+    V = (V @ op.torch.from_numpy(np.ascontiguousarray(Q)).to(V.device)).contiguous()
+    return V, (float(lam[0]), float(lam[1]))
+
+
+def _gap_grad_stats(op: SpectralOperator, n_components: int, seed: int, v0=None):
+    torch = op.torch
+    gap, evals, U, st = _gap_stats(op, n_components, seed, v0)
+    k = n_components
+    V, (l0, l1) = _pair_f64(op, U, evals, k)
+    sg = torch.tensor([np.sign(l0) or 1.0, -(np.sign(l1) or 1.0)], dtype=torch.float64, device=V.device)
     G = (V * sg).contiguous()
-    return gap, op.vjp_w(V, G), evals, U
+    return abs(l0) - abs(l1), op.vjp_w(V, G), evals, U, st
 
 
 def tune_feature_weights(op: SpectralOperator, n_components: int = 30, steps: int = 10,
@@ -116,16 +162,21 @@ def tune_feature_weights(op: SpectralOperator, n_components: int = 30, steps: in
     """Gradient ascent on the eigengap |lambda_k| - |lambda_{k+1}| over w >= 0.
 
     Each step moves w by step_size * mean(w) * g / max|g| and is kept only if
-    the gap grows (otherwise the step size is halved), so the returned gap is
-    never below the one of the initial weights. Returns a TuneResult: the
-    weights, the (gap, step_size) history of the accepted iterates, and the
-    n_components + 1 eigenpairs of largest magnitude for the returned weights,
-    so a caller can embed with them without another eigensolve. `op` is left
-    prepared for the returned weights. Each trial eigensolve starts from the
-    eigenvectors of the current weights."""
+    the gap grows by more than its uncertainty (otherwise the step size is
+    halved): the gaps compared are FP64 Rayleigh quotients of the two Lanczos
+    vectors, and the margin is their error bound (_checked_gap), so an accepted
+    step is a real increase, not rounding of the FP32 products. The returned gap
+    is never below the one of the initial weights. Returns a TuneResult: the
+    weights, the (gap, step_size) history of the accepted iterates (FP64 gaps),
+    and the n_components + 1 eigenpairs of largest magnitude for the returned
+    weights, so a caller can embed with them without another eigensolve. `op` is
+    left prepared for the returned weights. Each trial eigensolve starts from
+    the eigenvectors of the current weights."""
     torch = op.torch
+    k = n_components
     w = op.w.clone()
-    gap, g, evals, U = spectral_gap_grad(op, n_components, seed)
+    _, g, evals, U, st = _gap_grad_stats(op, n_components, seed)
+    gap, unc = _checked_gap(op, evals, U, st, k)
     history = [(gap, 0.0)]
     for it in range(steps):
         if step_size < min_step:
@@ -139,14 +190,15 @@ def tune_feature_weights(op: SpectralOperator, n_components: int = 30, steps: in
         try:
             op.set_weights(trial)
             if last:  # no further step: the gradient at the trial point is not needed
-                new_gap, new_evals, new_U = spectral_gap(op, n_components, seed, v0=U)
+                _, new_evals, new_U, new_st = _gap_stats(op, n_components, seed, v0=U)
                 new_g = None
             else:
-                new_gap, new_g, new_evals, new_U = spectral_gap_grad(op, n_components, seed, v0=U)
+                _, new_g, new_evals, new_U, new_st = _gap_grad_stats(op, n_components, seed, v0=U)
+            new_gap, new_unc = _checked_gap(op, new_evals, new_U, new_st, k)
         except _lib.PanicException:
-            new_gap = -np.inf
-        if new_gap > gap:
-            w, gap, g, evals, U = trial, new_gap, new_g, new_evals, new_U
+            new_gap, new_unc = -np.inf, 0.0
+        if new_gap - new_unc > gap + unc:
+            w, gap, unc, g, evals, U = trial, new_gap, new_unc, new_g, new_evals, new_U
             history.append((gap, step_size))
             step_size *= 1.5
         else:

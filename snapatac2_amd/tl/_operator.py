@@ -14,6 +14,8 @@ from .. import _lib
 from .._device import DeviceCSR
 
 TRANSPOSE_CHUNK = 1 << 28  # entries sorted per transpose chunk
+BLOCK_TARGET = 512  # blocks per direction of the block-tiled layout (about two per CU)
+DEFAULT_TILED = "panel"  # layout of large operators
 
 
 def _to_device_csr(X, torch) -> DeviceCSR:
@@ -72,7 +74,13 @@ def select_columns(X: DeviceCSR, cols: np.ndarray | None) -> DeviceCSR:
 class SpectralOperator:
     """Holds CSR/CSC counts, weights, scales and degrees of A on the device."""
 
-    def __init__(self, X: DeviceCSR, feature_weights=None, use_panels: bool | None = None):
+    def __init__(self, X: DeviceCSR, feature_weights=None, use_panels: bool | None = None,
+                 layout: str | None = None):
+        """layout: 'block' (LDS-tiled products, b <= 16 per launch), 'panel'
+        (L2-sized panels, b <= 32) or 'csr' (row-parallel products); by default
+        DEFAULT_TILED when either dense operand spans more than one 32,768-row
+        panel, else 'csr'. use_panels=True / False stands for the default tiled layout /
+        'csr'."""
         torch = _lib.require_gpu()
         L = _lib.lib()
         self.torch = torch
@@ -115,13 +123,24 @@ class SpectralOperator:
         # panel layouts (dense operands larger than one L2-sized panel)
         pc = int(L.snapgpu_panel_count(m))
         pr = int(L.snapgpu_panel_count(n))
-        self.use_panels = (pc > 1 or pr > 1) if use_panels is None else bool(use_panels)
+        if layout is None:
+            tiled = (pc > 1 or pr > 1) if use_panels is None else bool(use_panels)
+            layout = DEFAULT_TILED if tiled else "csr"
+        if layout not in ("block", "panel", "csr"):
+            raise ValueError("layout must be 'block', 'panel' or 'csr'")
+        self.layout = layout
+        self.use_panels = layout != "csr"
         self._panel_keep = []
-        if self.use_panels:
+        if layout == "panel":
             self.panel_csr = self._build_panel(X.indptr, X.indices, X.data, n, m, X.nnz)
             self.panel_csc = self._build_panel(self.colptr, self.rowidx, self.cval, m, n, X.nnz)
             self._pws_b = int(L.snapgpu_spectral_apply_panel_workspace_bytes(
                 n, m, self.panel_csr.n_pairs, self.panel_csc.n_pairs))
+            self._pws = torch.empty(self._pws_b, dtype=torch.uint8, device=dev)
+        elif layout == "block":
+            self.block_csr = self._build_block(X.indptr, X.indices, X.data, n, m, X.nnz)
+            self.block_csc = self._build_block(self.colptr, self.rowidx, self.cval, m, n, X.nnz)
+            self._pws_b = int(L.snapgpu_spectral_apply_block_workspace_bytes(n, m))
             self._pws = torch.empty(self._pws_b, dtype=torch.uint8, device=dev)
         self._ws_b = int(L.snapgpu_spectral_apply_workspace_bytes(n, m))
         self._ws = torch.empty(self._ws_b, dtype=torch.uint8, device=dev)
@@ -160,6 +179,40 @@ class SpectralOperator:
                                 cstart.data_ptr(), cpair.data_ptr(), cpanel.data_ptr(), NC.value,
                                 rp_ptr.data_ptr(), rp_ids.data_ptr(), K.value, nrows, dense_rows,
                                 packed.value, self.value_type)
+
+    def _build_block(self, ptr, idx, val, nrows: int, dense_rows: int, nnz: int,
+                     target_blocks: int = BLOCK_TARGET):
+        """Block-tiled copy of one storage direction (see snapgpu_block_plan/build)."""
+        torch = self.torch
+        L = _lib.lib()
+        dev = ptr.device
+        st = _lib.stream_ptr()
+        pws_b = int(L.snapgpu_block_plan_workspace_bytes(nrows))
+        pws = torch.empty(pws_b, dtype=torch.uint8, device=dev)
+        nb, ts, packed = C.c_int64(), C.c_int64(), C.c_int32()
+        _lib.check(L.snapgpu_block_plan(ptr.data_ptr(), idx.data_ptr(), val.data_ptr(), nrows, dense_rows, nnz,
+                                        self.value_type, target_blocks, pws.data_ptr(), pws_b, C.byref(nb),
+                                        C.byref(ts), C.byref(packed), st))
+        P = -(-dense_rows // int(L.snapgpu_block_panel_rows()))
+        i32 = dict(dtype=torch.int32, device=dev)
+        ent = torch.empty(max(nnz, 1), **i32)
+        val_out = None if packed.value else torch.empty(max(nnz, 1), **i32)
+        tile_start = torch.empty(nb.value * P + 1, dtype=torch.int64, device=dev)
+        row0 = torch.empty(nb.value + 1, **i32)
+        out_row = torch.empty(nrows, **i32)
+        vslot = torch.empty(nrows, **i32)
+        nslots = torch.empty(max(nb.value, 1), **i32)
+        bws_b = int(L.snapgpu_block_build_workspace_bytes(ts.value))
+        bws = torch.empty(bws_b, dtype=torch.uint8, device=dev)
+        _lib.check(L.snapgpu_block_build(
+            ptr.data_ptr(), idx.data_ptr(), val.data_ptr(), nrows, dense_rows, nnz, packed.value, pws.data_ptr(),
+            pws_b, nb.value, ts.value, bws.data_ptr(), bws_b, ent.data_ptr(), _lib.ptr(val_out),
+            tile_start.data_ptr(), row0.data_ptr(), out_row.data_ptr(), vslot.data_ptr(), nslots.data_ptr(), st))
+        del pws, bws
+        self._panel_keep.append((ent, val_out, tile_start, row0, out_row, vslot, nslots))
+        return _lib.BlockLayout(ent.data_ptr(), _lib.ptr(val_out), tile_start.data_ptr(), row0.data_ptr(),
+                                out_row.data_ptr(), vslot.data_ptr(), nslots.data_ptr(), nb.value, P, nrows,
+                                dense_rows, nnz, packed.value, self.value_type)
 
     def _load_weights(self, w) -> None:
         torch = self.torch
@@ -220,15 +273,21 @@ class SpectralOperator:
         L = _lib.lib()
         X = self.X
         b = V.shape[1] if b is None else b
-        if b > 32:
-            for c0 in range(0, b, 32):
-                self.apply(V[:, c0:c0 + 32], Y[:, c0:c0 + 32])
+        wmax = 16 if self.layout == "block" else 32
+        if b > wmax:
+            for c0 in range(0, b, wmax):
+                self.apply(V[:, c0:c0 + wmax], Y[:, c0:c0 + wmax])
             return
         ev = None
         if self.events is not None:
             ev = (self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True))
             ev[0].record()
-        if self.use_panels:
+        if self.layout == "block":
+            _lib.check(L.snapgpu_spectral_apply_block(
+                C.byref(self.block_csr), C.byref(self.block_csc), self.r32.data_ptr(),
+                self.w2.data_ptr(), self.dinv.data_ptr(), V.data_ptr(), V.stride(0), b, Y.data_ptr(),
+                Y.stride(0), self._pws.data_ptr(), self._pws_b, _lib.stream_ptr()))
+        elif self.layout == "panel":
             _lib.check(L.snapgpu_spectral_apply_panel(
                 C.byref(self.panel_csr), C.byref(self.panel_csc), self.r32.data_ptr(),
                 self.w2.data_ptr(), self.dinv.data_ptr(), V.data_ptr(), V.stride(0), b, Y.data_ptr(),
@@ -255,3 +314,33 @@ class SpectralOperator:
     def degree(self):
         """d = X~ X~^T 1 - 1 (what spectral_mf returns for Nystrom, embedding.rs:189-190)."""
         return 1.0 / self.dinv
+
+    def apply_f64(self, U):
+        """A U for U (n x b FP64) with both sparse products in FP64
+        (snapgpu_csr_spmm_scaled_f64 on the CSC, then on the CSR), for checks of
+        eigenvector estimates that the FP32 products' rounding (about 1e-7 of
+        |lambda_1|) would blur. None for float-valued matrices (the FP64
+        products read integer counts)."""
+        if self.value_type != 0:
+            return None
+        torch = self.torch
+        L = _lib.lib()
+        st = _lib.stream_ptr()
+        U = U.to(torch.float64).contiguous()
+        b = U.shape[1]
+        w2 = (self.w * self.w).contiguous()
+        Z = torch.empty((self.m, b), dtype=torch.float64, device=U.device)
+        _lib.check(L.snapgpu_csr_spmm_scaled_f64(self.colptr.data_ptr(), self.rowidx.data_ptr(),
+                                                 self.cval.data_ptr(), self.m, w2.data_ptr(),
+                                                 self.r64.data_ptr(), U.data_ptr(), b, b, Z.data_ptr(), b, st))
+        Y = torch.empty_like(U)
+        _lib.check(L.snapgpu_csr_spmm_scaled_f64(self.X.indptr.data_ptr(), self.X.indices.data_ptr(),
+                                                 self.X.data.data_ptr(), self.n, self.r64.data_ptr(), 0,
+                                                 Z.data_ptr(), b, b, Y.data_ptr(), b, st))
+        return Y - self.dinv[:, None] * U
+
+    def rayleigh_f64(self, U):
+        """Rayleigh quotients u^T A u of the columns of U with the FP64 products
+        of apply_f64 (None for float-valued matrices)."""
+        AU = self.apply_f64(U)
+        return None if AU is None else (U.to(AU.dtype) * AU).sum(0)

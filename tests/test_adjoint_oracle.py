@@ -90,3 +90,26 @@ def test_torch_tensor_forms():
     cf = adj.closed_form_grad(X, w, V, G)
     np.testing.assert_allclose(adj.closed_form_grad_tensors(T, wt, Vt, Gt).numpy(), cf,
                                rtol=1e-10, atol=1e-12 * np.abs(cf).max())
+
+
+def test_chunked_counts_forward_and_idf():
+    """ChunkedCounts (the FP64 checker of the 1M-cell tests) over chunks of 37
+    entries equals the forward, the column sums and document frequencies equal
+    scipy's, and idf_from_df equals oracle.spectral.idf (clamps, all-equal rule)."""
+    import torch
+    X, w, V, G = _problem(seed=21)
+    X = sp.csr_matrix(X, dtype=np.uint32)
+    T = adj.ChunkedCounts(torch.from_numpy(X.indptr.astype(np.int64)), torch.from_numpy(X.indices.astype(np.int32)),
+                          torch.from_numpy(X.data.view(np.int32)), X.shape, chunk_entries=37)
+    assert len(T.bounds) > 10
+    wt, Vt = torch.tensor(w), torch.tensor(V)
+    np.testing.assert_allclose(T.apply(wt, Vt).numpy(), adj.apply(X.astype(np.float64), w, V),
+                               rtol=1e-11, atol=1e-12)
+    np.testing.assert_array_equal(T.col_sums().numpy(), np.ravel(X.sum(axis=0)).astype(np.float64))
+    np.testing.assert_array_equal(T.df().numpy(), np.diff(X.tocsc().indptr))
+    np.testing.assert_allclose(adj.idf_from_df(T.df(), X.shape[0]).numpy(), osp.idf(X), rtol=1e-15)
+    n = 10
+    full = torch.full((5,), n)  # every column in every row: all equal -> 1
+    assert torch.equal(adj.idf_from_df(full, n), torch.ones(5, dtype=torch.float64))
+    got = adj.idf_from_df(torch.tensor([0, n, 2]), n).numpy()
+    np.testing.assert_allclose(got, [np.log(n / 1.0), np.log(n / (n - 1.0)), np.log(n / 2.0)], rtol=1e-15)

@@ -126,3 +126,41 @@ def test_chunk_skipping_is_exact(torch_gpu, n_clusters, spread):
     assert st["chunk_visits"] <= st["chunk_pairs"]
     if n_clusters == 40:
         assert st["chunk_visits"] < 0.7 * st["chunk_pairs"], st
+
+
+# --- beyond d = 30 and k = 100: the exact all-pairs path ------------------------
+
+@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+@pytest.mark.parametrize("n,d,k", [(4000, 50, 50), (3000, 20, 150), (2500, 64, 150), (300, 31, 101),
+                                   (200, 1, 199), (20_000, 31, 101)])
+def test_wide_rows_and_many_neighbours(torch_gpu, precision, n, d, k):
+    """knn.rs:9-32 has no limit on d or k: n_comps = 50 or n_neighbors = 150 give
+    the FP64 brute-force sets and bit-identical distances (20,000 points take
+    several query blocks of the scan)."""
+    from snapatac2_amd.pp._knn import nearest_neighbour_graph
+    X = clustered(n, d, seed=n + d + k)
+    g, st = nearest_neighbour_graph(X, k, precision=precision, return_stats=True)
+    _check(g, X, k)
+    assert st["exact_scan_rows"] == n
+
+
+def test_wide_ties_and_duplicates(torch_gpu):
+    """Integer lattice in 40 dimensions plus repeated rows: exact ties are ordered
+    by neighbour index, as in the oracle."""
+    from snapatac2_amd.pp._knn import nearest_neighbour_graph
+    rng = np.random.default_rng(2)
+    X = rng.integers(0, 2, (1200, 40)).astype(np.float64)
+    X[600:700] = X[0]
+    _check(nearest_neighbour_graph(X, 120), X, 120)
+
+
+def test_knn_dropin_with_50_components(torch_gpu):
+    import snapatac2_amd as snap
+    from snapatac2_amd.anndata import AnnData
+    X = clustered(3000, 50, seed=4)
+    ad = AnnData(n_obs=3000, n_vars=0)
+    ad.obsm["X_spectral"] = X
+    snap.pp.knn(ad, n_neighbors=150)
+    _check(ad.obsp["distances"], X, 150)
+    g = snap.pp.knn(ad, n_neighbors=30, use_dims=45, inplace=False)
+    _check(g, np.ascontiguousarray(X[:, :45]), 30)

@@ -45,3 +45,14 @@ def test_host_only_entry_points():
     assert L.snapgpu_tile_table_pack(lens.ctypes.data, excl.ctypes.data, 3, 0,
                                      buf.ctypes.data, ctypes.byref(nb)) == _lib.ERR_ARG
     assert b"bin_size" in L.snapgpu_last_error()
+
+
+def test_probe_library_is_separate():
+    """The measurement aids live in libsnapgpu_probe.so, not in the product ABI."""
+    from snapatac2_amd.build import PROBE_LIB
+    probe = ctypes.CDLL(str(PROBE_LIB))
+    for name in ("snapgpu_probe_last_error", "snapgpu_probe_copy_bytes", "snapgpu_probe_gather"):
+        assert hasattr(probe, name)
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    for name in ("snapgpu_copy_bytes", "snapgpu_gather_probe", "snapgpu_probe_gather"):
+        assert not hasattr(lib, name)

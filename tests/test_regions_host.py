@@ -50,3 +50,64 @@ def test_parse_regions_distinct_first_occurrence():
     for bad in ["chr1:5", "chr1-5-10", "chr1:-5-10", "chr1:a-b"]:
         with pytest.raises(_lib.PanicException):
             parse_regions([bad])
+
+
+# --- select_features' whitelist / blacklist (intersect_bed) ----------------
+
+from oracle import peaks as opeaks  # noqa: E402
+from snapatac2_amd.pp._regions import parse_range_names, read_bed3  # noqa: E402
+
+BED = [("chr1", 100, 200), ("chr1", 500, 501), ("chr2", 0, 10), ("chrX", 1000, 2000)]
+
+
+@pytest.mark.parametrize("name,expect", [
+    ("chr1:0-100", False),     # ends where the region starts (half-open)
+    ("chr1:0-101", True),
+    ("chr1:200-300", False),   # starts where the region ends
+    ("chr1:199-200", True),
+    ("chr1:120-130", True),    # inside
+    ("chr1:50-600", True),     # contains both chr1 regions
+    ("chr1:501-502", False),
+    ("chr1:500-500", False),   # empty query at a region's start
+    ("chr1:150-150", True),    # empty query strictly inside
+    ("chr2:9-10", True),
+    ("chr3:100-200", False),   # chromosome without regions
+    ("chrX:1999-5000", True),
+])
+def test_oracle_intersect_bed_hand_cases(name, expect):
+    assert opeaks.intersect_bed([name], BED)[0] == expect
+
+
+def test_read_bed3_plain_gzip_and_errors(tmp_path):
+    import gzip
+    text = "chr1\t100\t200\tname\t0\t+\nchr2\t0\t10\n\nchrX\t1000\t2000\n"
+    p = tmp_path / "a.bed"
+    p.write_text(text)
+    g = tmp_path / "a.bed.gz"
+    with gzip.open(g, "wt") as f:
+        f.write(text)
+    want = [("chr1", 100, 200), ("chr2", 0, 10), ("chrX", 1000, 2000)]
+    assert read_bed3(p) == want and read_bed3(g) == want
+    bad = tmp_path / "b.bed"
+    bad.write_text("chr1\t100\n")
+    with pytest.raises(_lib.PanicException):
+        read_bed3(bad)
+    bad.write_text("track name=x\n")
+    with pytest.raises(_lib.PanicException):
+        read_bed3(bad)
+
+
+def test_parse_range_names():
+    ch, c, s, e = parse_range_names(["chr1:1-500", "HLA:A:5-9", " chr2:0-3 ", "chr1:7-8"])
+    assert [ch[k] for k in c] == ["chr1", "HLA:A", "chr2", "chr1"]
+    assert list(s) == [1, 5, 0, 7] and list(e) == [500, 9, 3, 8]
+    with pytest.raises(_lib.PanicException):
+        parse_range_names(["chr1_100_200"])
+
+
+def test_bin_index_ranges_equal_the_names():
+    from snapatac2_amd.genome import BinIndex
+    bi = BinIndex({"chr1": 1234, "chr10": 500, "chr2": 1}, 500)
+    ch, c, s, e = bi.ranges()
+    assert [f"{ch[k]}:{a}-{b}" for k, a, b in zip(c, s, e)] == list(bi.to_numpy())
+    assert list(bi.to_numpy()) == ["chr1:0-500", "chr1:500-1000", "chr1:1000-1234", "chr10:0-500", "chr2:0-1"]

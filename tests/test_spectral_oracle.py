@@ -142,3 +142,31 @@ def test_nystrom_oracle_pin_rejects_inverted_scaling(monkeypatch):
     v, q = osp.nystrom(X, rows, 6)
     _, oq = osp.orthogonalize(v, q)
     assert osp.subspace_angle(oq.real[:, :3], eu[:, :3]) > 0.2
+
+
+def test_subspace_sin_known_rotation():
+    """sin of the largest principal angle: a basis rotated by theta in one
+    direction gives sin(theta), also at 1e-9 rad where arccos of the cosine
+    loses its digits."""
+    rng = np.random.default_rng(0)
+    Q, _ = np.linalg.qr(rng.standard_normal((200, 6)))
+    for th in (1e-9, 1e-6, 1e-3, 0.3):
+        V = Q[:, :5].copy()
+        V[:, 2] = np.cos(th) * Q[:, 2] + np.sin(th) * Q[:, 5]
+        assert abs(osp.subspace_sin(Q[:, :5], V @ rng.standard_normal((5, 5))) - np.sin(th)) < 1e-12 + 1e-9 * th
+
+
+def test_eigen_device_equals_scipy_restatement():
+    """eigen_device (the reference's eigsh call on the torch FP64 operator)
+    gives the eigenpairs of the scipy restatement on the same counts."""
+    import torch
+    from oracle import adjoint as oadj
+    X = small_counts(seed=4)
+    T = oadj.ChunkedCounts(torch.from_numpy(X.indptr.astype(np.int64)), torch.from_numpy(X.indices.astype(np.int32)),
+                           torch.from_numpy(X.data.view(np.int32)), X.shape, chunk_entries=500)
+    w = torch.from_numpy(osp.idf(X))
+    ev, U, nmv = osp.eigen_device(T, w, 8, seed=0)
+    ev2, U2 = osp.spectral_embedding(X, None, 8, 0)
+    np.testing.assert_allclose(ev, ev2, rtol=1e-10, atol=1e-12)
+    assert osp.subspace_sin(U, U2) < 1e-8
+    assert nmv > 8

@@ -160,3 +160,109 @@ def test_select_features_matches_numpy(torch_gpu):
     sel[_find_most_accessible_features(count, 0.005, 0.005, 20000)] = True
     assert np.array_equal(ad.var["selected"].to_numpy(), sel)
     assert sel.sum() == 20000
+
+
+def _shuffled(rng, cells):
+    out = []
+    for c in cells:
+        c = list(c)
+        rng.shuffle(c)
+        out.append(c)
+    return out
+
+
+@pytest.mark.parametrize("strategy", ["insertion", "fragment", "paired-insertion", "single"])
+def test_rows_out_of_order_give_the_sorted_result(torch_gpu, strategy):
+    """The reference's per-row BTreeMap accepts fragments in any order
+    (data_iter.rs:362-411): shuffled rows give the matrix of the sorted rows.
+    The tile pass reports them (SNAPGPU_ERR_UNSORTED) and the host sorts a copy;
+    the caller's arrays stay as they were."""
+    from test_tile_oracle import random_cells
+    from snapatac2_amd.pp import make_tile_matrix
+    rng = np.random.default_rng(23)
+    cs = {"chr1": 50000, "chr10": 12345, "chr2": 30000, "chrM": 3000, "chrX": 25000}
+    cells = random_cells(rng, cs, 200, max_frags=300, max_len=1200)
+    shuf = _shuffled(rng, cells)
+    shuf[3] = cells[3]  # sorted rows among unsorted ones
+    paired = strategy != "single"
+    fr, cp, pos, ln = _frags(torch_gpu, shuf, paired)
+    before = fr.pos.clone()
+    X, _ = make_tile_matrix(fr, cs, 500, ["chrM"], None, None, strategy if paired else "paired-insertion")
+    o = otile.tile_matrix(cp, pos, ln, cs, 500, ["chrM"], None, None, strategy)
+    _same(X.to_scipy(), o)
+    assert torch_gpu.equal(fr.pos, before)
+    # the oracle itself is order-independent
+    sp_, spos, sln = _frags(torch_gpu, cells, paired)[1:]
+    _same(o, otile.tile_matrix(sp_, spos, sln, cs, 500, ["chrM"], None, None, strategy))
+
+
+def test_rows_out_of_order_chunked(torch_gpu):
+    from snapatac2_amd import synth
+    from snapatac2_amd.genome import synthetic_genome
+    from snapatac2_amd.pp._basic import make_tile_matrix_chunked
+    cs = synthetic_genome()
+    rng = np.random.default_rng(4)
+    chunks, hs = [], []
+    first = 0
+    for k, n in enumerate([300, 200, 250]):
+        fr = synth.generate_chunk(synth.GeneratorTables(0), first, n)
+        h = fr.to_scipy()
+        if k != 1:  # chunk 1 stays sorted
+            for r in rng.choice(n, 40, replace=False):
+                a, b = h.indptr[r], h.indptr[r + 1]
+                perm = rng.permutation(b - a)
+                h.indices[a:b] = h.indices[a:b][perm]
+                h.data[a:b] = h.data[a:b][perm]
+            fr = type(fr).from_scipy(h, paired=True)
+        chunks.append((first, fr))
+        hs.append(h)
+        first += n
+    X, _ = make_tile_matrix_chunked(chunks, cs, 500, ["chrM", "chrY"])
+    import scipy.sparse as sp
+    H = sp.vstack(hs, format="csr")
+    o = otile.tile_matrix(H.indptr, H.indices, H.data, cs, 500, ["chrM", "chrY"])
+    _same(X.to_scipy(), o)
+
+
+def test_sort_rows_entry_and_unsorted_status(torch_gpu):
+    """snapgpu_frag_sort_rows orders each row by (position, value); the count pass
+    returns SNAPGPU_ERR_UNSORTED for a row whose positions go down, also when the
+    decrease crosses a 64-fragment tile boundary."""
+    import ctypes
+    from snapatac2_amd import _lib
+    torch = torch_gpu
+    L = _lib.lib()
+    rng = np.random.default_rng(9)
+    cells = [[(int(p), int(l)) for p, l in zip(np.sort(rng.integers(0, 40000, m)), rng.integers(1, 500, m))]
+             for m in (0, 1, 64, 65, 130, 500)]
+    cells[4][64], cells[4][63] = cells[4][63], cells[4][64]  # swap across the tile boundary
+    cells[4][63] = (cells[4][64][0] + 1, 7)
+    cs = {"chr1": 50000}
+    fr, cp, pos, ln = _frags(torch, cells, True)
+    tab_h = np.zeros(L.snapgpu_tile_table_bytes(1), dtype=np.uint8)
+    nb = ctypes.c_uint64()
+    _lib.check(L.snapgpu_tile_table_pack(np.asarray([50000], np.uint64).ctypes.data,
+                                         np.zeros(1, np.uint8).ctypes.data, 1, 500, tab_h.ctypes.data,
+                                         ctypes.byref(nb)))
+    tab = torch.from_numpy(tab_h).cuda()
+    rows = torch.zeros(len(cells), dtype=torch.int64, device="cuda")
+    status = torch.zeros(8, dtype=torch.int32, device="cuda")
+    level = ctypes.c_int32(0)
+    rc = L.snapgpu_tile_count(fr.cell_ptr.data_ptr(), fr.pos.data_ptr(), fr.values.data_ptr(), len(cells),
+                              tab.data_ptr(), 1, 500, 2, 0, 0xFFFFFFFF, rows.data_ptr(), 0, status.data_ptr(),
+                              ctypes.byref(level), _lib.stream_ptr())
+    assert rc == _lib.ERR_UNSORTED and b"cell 4" in L.snapgpu_last_error()
+    F = fr.n_fragments
+    wsb = int(L.snapgpu_frag_sort_rows_workspace_bytes(len(cells), F))
+    ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+    _lib.check(L.snapgpu_frag_sort_rows(fr.cell_ptr.data_ptr(), len(cells), F, fr.pos.data_ptr(),
+                                        fr.values.data_ptr(), ws.data_ptr(), wsb, _lib.stream_ptr()))
+    want = [sorted(c) for c in cells]
+    gp = fr.pos.cpu().numpy().view(np.uint32)
+    gl = fr.values.cpu().numpy().view(np.uint32)
+    got = [list(zip(gp[a:b].tolist(), gl[a:b].tolist())) for a, b in zip(cp[:-1], cp[1:])]
+    assert got == want
+    level.value = 0
+    _lib.check(L.snapgpu_tile_count(fr.cell_ptr.data_ptr(), fr.pos.data_ptr(), fr.values.data_ptr(), len(cells),
+                                    tab.data_ptr(), 1, 500, 2, 0, 0xFFFFFFFF, rows.data_ptr(), 0,
+                                    status.data_ptr(), ctypes.byref(level), _lib.stream_ptr()))
