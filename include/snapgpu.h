@@ -415,6 +415,22 @@ int64_t snapgpu_gemm_tn_workspace_bytes(int64_t n, int32_t ka, int32_t kb);
 int snapgpu_gemm_tn_f64(const double* d_A, int64_t lda, const double* d_B, int64_t ldb,
                         int64_t n, int32_t ka, int32_t kb, double* d_C, void* d_ws,
                         int64_t ws_bytes, void* stream);
+
+/* Allocation-free form of snapgpu_gemm_tn_f64. It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The per-chunk partial products go to a static device array of
+ * SNAPGPU_GEMM_TN_SCRATCH_BYTES held by the library on every GPU it runs on
+ * (until the process ends) instead of a caller workspace; the call allocates
+ * nothing. It fails with SNAPGPU_ERR_CAPACITY when
+ * snapgpu_gemm_tn_workspace_bytes(n, ka, kb) - 256 exceeds that size (at most
+ * 512 row chunks x ka x kb doubles: ka * kb <= 8192 always fits). Every call on
+ * one GPU shares that GPU's array, so calls that overlap in time on one GPU
+ * (other streams, other threads) race; calls ordered on one stream do not.
+ * Same result, bit for bit, as snapgpu_gemm_tn_f64. */
+#define SNAPGPU_GEMM_TN_SCRATCH_BYTES (32ll << 20)
+int snapgpu_gemm_tn_f64_noalloc(const double* d_A, int64_t lda, const double* d_B, int64_t ldb,
+                                int64_t n, int32_t ka, int32_t kb, double* d_C, void* stream);
 /* C[n x kb] = beta*C + alpha*A[n x ka] B[ka x kb]; kb <= 32. */
 int snapgpu_gemm_nn_f64(const double* d_A, int64_t lda, const double* d_B, int64_t n, int32_t ka,
                         int32_t kb, double alpha, double beta, double* d_C, int64_t ldc,

@@ -77,6 +77,7 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_spectral_apply_block": (C.c_int, [_p, _p, _p, _p, _p, _p, _i64, _i32, _p, _i64, _p, _i64, _p]),
     "snapgpu_gemm_tn_workspace_bytes": (_i64, [_i64, _i32, _i32]),
     "snapgpu_gemm_tn_f64": (C.c_int, [_p, _i64, _p, _i64, _i64, _i32, _i32, _p, _p, _i64, _p]),
+    "snapgpu_gemm_tn_f64_noalloc": (C.c_int, [_p, _i64, _p, _i64, _i64, _i32, _i32, _p, _p]),
     "snapgpu_gemm_nn_f64": (C.c_int, [_p, _i64, _p, _i64, _i32, _i32, C.c_double, C.c_double,
                                       _p, _i64, _p]),
     "snapgpu_random_f64": (C.c_int, [_u64, _i64, _p, _p]),

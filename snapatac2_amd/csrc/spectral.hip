@@ -20,6 +20,8 @@
 // summation order everywhere (no floating-point atomics): results are
 // bitwise reproducible.
 #include <hipcub/hipcub.hpp>
+#include <mutex>
+#include <string>
 
 #include "spectral_common.h"
 
@@ -901,6 +903,77 @@ static inline unsigned grid_for(int64_t work, int per_block, int64_t cap = 1 << 
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, per_block), cap));
 }
 
+// --- C = A^T B over row chunks ------------------------------------------------
+// Up to 512 chunks of rows; each writes its ka x kb partial product, then the
+// partials are added in chunk order.
+static int64_t gemm_tn_chunks(int64_t n) {
+  return std::max<int64_t>(1, std::min<int64_t>(512, ceil_div(n, 4096)));
+}
+
+static int64_t gemm_tn_partial_bytes(int64_t n, int32_t ka, int32_t kb) {
+  return gemm_tn_chunks(n) * (int64_t)ka * kb * 8;
+}
+
+static int gemm_tn_launch(const double* d_A, int64_t lda, const double* d_B, int64_t ldb, int64_t n,
+                          int32_t ka, int32_t kb, double* d_C, double* part, hipStream_t st) {
+  const int64_t chunks = gemm_tn_chunks(n);
+  const int64_t rpc = ceil_div(std::max<int64_t>(n, 1), chunks);
+  // 16*TA columns of A per block: as few column groups as 128-wide ones need,
+  // each as narrow as covers ka
+  const int groups = (int)ceil_div(ka, 128);
+  int ta = (int)ceil_div(ka, 16 * groups);
+  if (ta > 1 && (ta & 1)) ++ta;
+  const int tc = kb <= 16 ? 1 : 2;
+  const int vec_ok = (lda % 2 == 0) && (((uintptr_t)d_A & 15) == 0);
+  const dim3 grid((unsigned)chunks, (unsigned)ceil_div(ka, 16 * ta));
+#define SG_TN(TA, TC)                                                                              \
+  gemm_tn_kernel<TA, TC><<<grid, 256, 0, st>>>(d_A, lda, d_B, ldb, n, ka, kb, rpc, vec_ok, part)
+#define SG_TN_TA(TC)      \
+  switch (ta) {           \
+    case 1: SG_TN(1, TC); break; \
+    case 2: SG_TN(2, TC); break; \
+    case 4: SG_TN(4, TC); break; \
+    case 6: SG_TN(6, TC); break; \
+    default: SG_TN(8, TC); break; \
+  }
+  if (tc == 1) {
+    SG_TN_TA(1)
+  } else {
+    SG_TN_TA(2)
+  }
+#undef SG_TN_TA
+#undef SG_TN
+  SG_LAUNCH_CHECK();
+  const int64_t count = (int64_t)ka * kb;
+  reduce_partials_kernel<<<(unsigned)ceil_div(count, 16), 256, 0, st>>>(part, chunks, count, d_C);
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
+// Static scratch of snapgpu_gemm_tn_f64_noalloc: one copy per GPU, held until the
+// process ends. In host code `scratch` only names the symbol; its device address
+// on the current GPU comes from hipGetSymbolAddress, looked up once per device
+// and kept in g_scratch_addr.
+static __device__ __align__(256) char scratch[SNAPGPU_GEMM_TN_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[kMaxDevices] = {};
+
+static int scratch_address(double** out) {
+  int dev = -1;
+  SG_HIP(hipGetDevice(&dev));
+  SG_ARG(dev >= 0 && dev < kMaxDevices, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_addr[dev] == nullptr) {
+    void* p = nullptr;
+    SG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SG_ARG(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_addr[dev] = p;
+  }
+  *out = (double*)g_scratch_addr[dev];
+  return SNAPGPU_OK;
+}
+
 extern "C" {
 
 int snapgpu_col_select_table(const int64_t* d_cols, int64_t k, int64_t m, uint32_t* d_sel, void* stream) {
@@ -1163,8 +1236,7 @@ int snapgpu_spectral_vjp_w(const int64_t* d_indptr, const int32_t* d_idx, const 
 }
 
 int64_t snapgpu_gemm_tn_workspace_bytes(int64_t n, int32_t ka, int32_t kb) {
-  const int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(512, ceil_div(n, 4096)));
-  return chunks * (int64_t)ka * kb * 8 + 256;
+  return gemm_tn_partial_bytes(n, ka, kb) + 256;
 }
 
 // C[ka x kb] = A[n x ka]^T B[n x kb] (row-major, leading dims lda/ldb).
@@ -1175,40 +1247,28 @@ int snapgpu_gemm_tn_f64(const double* d_A, int64_t lda, const double* d_B, int64
   SG_ARG(ka >= 1 && kb >= 1 && kb <= 32, "need 1 <= kb <= 32");
   SG_ARG(n >= 0 && lda >= ka && ldb >= kb, "need n >= 0, lda >= ka, ldb >= kb");
   SG_ARG(ws_bytes >= snapgpu_gemm_tn_workspace_bytes(n, ka, kb), "workspace too small");
-  hipStream_t st = as_stream(stream);
-  const int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(512, ceil_div(n, 4096)));
-  const int64_t rpc = ceil_div(std::max<int64_t>(n, 1), chunks);
-  double* part = (double*)d_ws;
-  // 16*TA columns of A per block: as few column groups as 128-wide ones need,
-  // each as narrow as covers ka
-  const int groups = (int)ceil_div(ka, 128);
-  int ta = (int)ceil_div(ka, 16 * groups);
-  if (ta > 1 && (ta & 1)) ++ta;
-  const int tc = kb <= 16 ? 1 : 2;
-  const int vec_ok = (lda % 2 == 0) && (((uintptr_t)d_A & 15) == 0);
-  const dim3 grid((unsigned)chunks, (unsigned)ceil_div(ka, 16 * ta));
-#define SG_TN(TA, TC)                                                                              \
-  gemm_tn_kernel<TA, TC><<<grid, 256, 0, st>>>(d_A, lda, d_B, ldb, n, ka, kb, rpc, vec_ok, part)
-#define SG_TN_TA(TC)      \
-  switch (ta) {           \
-    case 1: SG_TN(1, TC); break; \
-    case 2: SG_TN(2, TC); break; \
-    case 4: SG_TN(4, TC); break; \
-    case 6: SG_TN(6, TC); break; \
-    default: SG_TN(8, TC); break; \
+  return gemm_tn_launch(d_A, lda, d_B, ldb, n, ka, kb, d_C, (double*)d_ws, as_stream(stream));
+}
+
+// The same product with the partial products in the library's static scratch
+// array of this GPU (no workspace argument, no allocation).
+int snapgpu_gemm_tn_f64_noalloc(const double* d_A, int64_t lda, const double* d_B, int64_t ldb,
+                                int64_t n, int32_t ka, int32_t kb, double* d_C, void* stream) {
+  clear_error();
+  SG_ARG(ka >= 1 && kb >= 1 && kb <= 32, "need 1 <= kb <= 32");
+  SG_ARG(n >= 0 && lda >= ka && ldb >= kb, "need n >= 0, lda >= ka, ldb >= kb");
+  const int64_t need = gemm_tn_partial_bytes(n, ka, kb);
+  if (need > SNAPGPU_GEMM_TN_SCRATCH_BYTES) {
+    set_error("capacity: snapgpu_gemm_tn_f64_noalloc needs " + std::to_string(need) +
+              " bytes of partial products (512 row chunks x ka x kb doubles at most), more than its " +
+              std::to_string((long long)SNAPGPU_GEMM_TN_SCRATCH_BYTES) +
+              "-byte static scratch; call snapgpu_gemm_tn_f64 with a workspace instead");
+    return SNAPGPU_ERR_CAPACITY;
   }
-  if (tc == 1) {
-    SG_TN_TA(1)
-  } else {
-    SG_TN_TA(2)
-  }
-#undef SG_TN_TA
-#undef SG_TN
-  SG_LAUNCH_CHECK();
-  const int64_t count = (int64_t)ka * kb;
-  reduce_partials_kernel<<<(unsigned)ceil_div(count, 16), 256, 0, st>>>(part, chunks, count, d_C);
-  SG_LAUNCH_CHECK();
-  return SNAPGPU_OK;
+  double* part = nullptr;
+  const int rc = scratch_address(&part);
+  if (rc) return rc;
+  return gemm_tn_launch(d_A, lda, d_B, ldb, n, ka, kb, d_C, part, as_stream(stream));
 }
 
 // C[n x kb] = beta*C + alpha * A[n x ka] B[ka x kb] (B row-major, contiguous).

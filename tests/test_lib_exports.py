@@ -47,6 +47,17 @@ def test_host_only_entry_points():
     assert b"bin_size" in L.snapgpu_last_error()
 
 
+def test_gemm_tn_noalloc_rejects_beyond_its_scratch():
+    """The allocation-free basis product checks its size on the host, before any
+    HIP call: 512 chunks x 320 x 32 doubles (40 MiB) exceed the 32 MiB array."""
+    L = _lib.lib()
+    rc = L.snapgpu_gemm_tn_f64_noalloc(None, 320, None, 32, 512 * 4096, 320, 32, None, None)
+    assert rc == _lib.ERR_CAPACITY
+    msg = L.snapgpu_last_error()
+    assert b"capacity" in msg and b"33554432" in msg
+    assert L.snapgpu_gemm_tn_f64_noalloc(None, 4, None, 40, 10, 4, 40, None, None) == _lib.ERR_ARG
+
+
 def test_probe_library_is_separate():
     """The measurement aids live in libsnapgpu_probe.so, not in the product ABI."""
     from snapatac2_amd.build import PROBE_LIB
