@@ -279,6 +279,26 @@ class ChunkedCounts:
         r = (u * a)[:, None]
         return r * self.mm((w * w)[:, None] * self.rmm(r * V)) - V / d[:, None]
 
+    def scaled_csr(self, w):
+        """(Xh, XhT, d): Xh = diag(u a) X diag(w) and its transpose as FP64 torch
+        sparse CSR tensors (torch's own sparse kernels then apply them), with
+        A(w) v = Xh (XhT v) - v / d. Needs the entries in memory three times over."""
+        torch = self.torch
+        u, d, a = self.scales(w)
+        r = u * a
+        rows, cols, vals = [], [], []
+        for rr, cc, v in self.chunks():
+            rows.append(rr)
+            cols.append(cc)
+            vals.append(v * r[rr] * w[cc])
+        rows, cols, vals = torch.cat(rows), torch.cat(cols), torch.cat(vals)
+        import warnings
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", UserWarning)  # torch marks sparse CSR as beta
+            Xh = torch.sparse_coo_tensor(torch.stack([rows, cols]), vals, (self.n, self.m)).coalesce().to_sparse_csr()
+            XhT = torch.sparse_coo_tensor(torch.stack([cols, rows]), vals, (self.m, self.n)).coalesce().to_sparse_csr()
+        return Xh, XhT, d
+
 
 def idf_from_df(df, n: int):
     """idf (embedding.rs:269-286) from the document frequencies (torch, FP64):

@@ -184,18 +184,19 @@ def eigen_device(T, w, k: int, seed: int = 0):
     products run on T's device through torch: the reference's own eigensolver
     call on an operator too large for the scipy restatement above. Returns
     (eigenvalues descending, eigenvectors n x k, number of matvecs)."""
+    import sys
     import torch
     from scipy.sparse.linalg import LinearOperator, eigsh
-    u, d, a = T.scales(w)
-    r = u * a
-    w2 = w * w
+    Xh, XhT, d = T.scaled_csr(w)
     count = [0]
 
     def f(v):
         count[0] += 1
-        x = torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64).reshape(-1)).to(r.device)
-        y = r * T.mm(w2 * T.rmm(r * x)) - x / d
-        return y.cpu().numpy()
+        if count[0] % 500 == 0:
+            print(f"eigen_device: {count[0]} matvecs", file=sys.stderr, flush=True)
+        x = torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 1)).to(d.device)
+        y = torch.sparse.mm(Xh, torch.sparse.mm(XhT, x)) - x / d[:, None]
+        return y.cpu().numpy().reshape(-1)
 
     n = T.n
     np.random.seed(seed)
