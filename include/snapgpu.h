@@ -356,13 +356,15 @@ int snapgpu_spectral_apply_panel(const snapgpu_panel_layout* csr, const snapgpu_
  * operand is swept in panels of snapgpu_block_panel_rows() rows, each read
  * once per block into LDS, from which every entry of the tile gathers its row.
  * A row's entries inside one panel are cut into pieces of at most 16, each
- * with its own accumulator slot, so the 256 four-lane groups of a workgroup
- * share a tile's entries evenly and every slot has one writer: fixed summation
- * order. Build: snapgpu_block_plan (row order, pieces, blocks; its workspace
- * keeps them), then allocate, then snapgpu_block_build.
+ * with its own accumulator slot; a tile stores its pieces by decreasing length,
+ * entry i of each piece in row i, so each thread of the workgroup reads one
+ * piece with loads coalesced across its wave, and every slot has one writer:
+ * fixed summation order. Build: snapgpu_block_plan (row order, pieces, blocks;
+ * its workspace keeps them), then allocate, then snapgpu_block_build.
  *   ent / val      n_entries uint32 (val only when not packed: counts >= 1024
  *                  or float values)
  *   tile_start     n_blocks * n_panels + 1 int64 (tile = (block, panel))
+ *   tile_off       n_blocks * n_panels * 16 uint16 (end of row i of each tile)
  *   row0           n_blocks + 1 int32 (first out position of each block)
  *   out_row        n_rows int32 (row of each out position)
  *   vslot          n_rows int32 (first slot of each out position in its block)
@@ -371,6 +373,7 @@ typedef struct snapgpu_block_layout {
   const uint32_t* ent;
   const uint32_t* val;
   const int64_t* tile_start;
+  const uint16_t* tile_off;
   const int32_t* row0;
   const int32_t* out_row;
   const int32_t* vslot;
@@ -398,8 +401,8 @@ int64_t snapgpu_block_build_workspace_bytes(int64_t table_size);
 int snapgpu_block_build(const int64_t* d_ptr, const int32_t* d_idx, const uint32_t* d_val, int64_t nrows,
                         int64_t dense_rows, int64_t nnz, int32_t packed, const void* d_plan_ws, int64_t plan_ws_bytes,
                         int64_t n_blocks, int64_t table_size, void* d_ws, int64_t ws_bytes, uint32_t* d_ent,
-                        uint32_t* d_val_out, int64_t* d_tile_start, int32_t* d_row0, int32_t* d_out_row,
-                        int32_t* d_vslot, int32_t* d_nslots, void* stream);
+                        uint32_t* d_val_out, int64_t* d_tile_start, uint16_t* d_tile_off, int32_t* d_row0,
+                        int32_t* d_out_row, int32_t* d_vslot, int32_t* d_nslots, void* stream);
 /* Y = A V as snapgpu_spectral_apply (b <= 16), over the two block layouts (csr:
  * rows = cells, dense_rows = features; csc: rows = features, dense_rows = cells). */
 int64_t snapgpu_spectral_apply_block_workspace_bytes(int64_t n, int64_t m);

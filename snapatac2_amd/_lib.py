@@ -72,7 +72,7 @@ SIGNATURES: dict[str, tuple] = {
                                      C.POINTER(_i64), C.POINTER(_i32), _p]),
     "snapgpu_block_build_workspace_bytes": (_i64, [_i64]),
     "snapgpu_block_build": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _i32, _p, _i64, _i64, _i64, _p, _i64, _p, _p,
-                                      _p, _p, _p, _p, _p, _p]),
+                                      _p, _p, _p, _p, _p, _p, _p]),
     "snapgpu_spectral_apply_block_workspace_bytes": (_i64, [_i64, _i64]),
     "snapgpu_spectral_apply_block": (C.c_int, [_p, _p, _p, _p, _p, _p, _i64, _i32, _p, _i64, _p, _i64, _p]),
     "snapgpu_gemm_tn_workspace_bytes": (_i64, [_i64, _i32, _i32]),
@@ -123,7 +123,8 @@ class PanelLayout(C.Structure):
 
 class BlockLayout(C.Structure):
     """snapgpu_block_layout (include/snapgpu.h): device pointers and sizes."""
-    _fields_ = [("ent", _p), ("val", _p), ("tile_start", _p), ("row0", _p), ("out_row", _p), ("vslot", _p),
+    _fields_ = [("ent", _p), ("val", _p), ("tile_start", _p), ("tile_off", _p), ("row0", _p), ("out_row", _p),
+                ("vslot", _p),
                 ("nslots", _p), ("n_blocks", _i64), ("n_panels", _i64), ("n_rows", _i64), ("dense_rows", _i64),
                 ("n_entries", _i64), ("packed", _i32), ("value_type", _i32)]
 

@@ -15,16 +15,18 @@
 // similar densities. A row's entries inside one panel (its "segment") are cut
 // into pieces of at most kPiece entries, and every piece of a row gets its own
 // accumulator slot (a row has as many slots as its longest segment has
-// pieces): a tile's work splits between the 256 four-lane groups of the
-// workgroup in equal entry ranges, each group taking the pieces that start in
-// its range, so no piece is shared and every slot has one writer per tile. The
-// slot sums are added per row in slot order at the end: fixed summation order,
-// no floating-point atomics, bitwise reproducible results.
+// pieces). Inside a tile the pieces are ordered by decreasing length, and
+// entry i of every piece long enough is stored in row i of the tile: thread k
+// of the workgroup owns the k-th piece, reads its entries with loads that are
+// coalesced across the wave, gathers all 16 columns of each entry's dense row
+// from LDS and adds the piece's sum to its slot. No piece is shared and every
+// slot has one writer per tile; the slot sums are added per row in slot order
+// at the end: fixed summation order, no floating-point atomics, bitwise
+// reproducible results.
 //
-// Entry encoding (packed, counts < 1024): bits 0-9 gather row within the
-// panel, bits 10-20 slot, bit 21 first entry of a piece, bits 22-31 count.
-// Unpacked (float values or large counts): the same low 22 bits, the raw
-// 32-bit value in `val`.
+// Entry encoding: bits 0-9 gather row within the panel, bits 10-19 the count
+// (packed layouts, counts < 1024), bits 20-30 the slot. Unpacked (float values
+// or large counts): the value in `val` instead of bits 10-19.
 #include <algorithm>
 #include <vector>
 
@@ -36,13 +38,12 @@ namespace snapgpu {
 namespace block {
 
 constexpr int kBR = 1024;          // gather rows per panel (10-bit local index)
-constexpr int kBC = 1280;          // accumulator slots per block (11-bit slot)
-constexpr int kPiece = 16;         // entries per piece at most
-constexpr int kThreads = 1024;     // 16 waves, 256 groups of 4 lanes
-constexpr int kGroups = kThreads / 4;
-constexpr uint32_t kFirst = 1u << 21;
-constexpr size_t kLds = (size_t)(kBR + kBC) * 16 * sizeof(float);  // 144 KB
-static_assert(kBC < 2048 && kBR == 1024, "slot and gather fields");
+constexpr int kBC = 1024;          // accumulator slots per block (= threads: one piece per thread)
+constexpr int kPiece = 16;         // entries per piece at most (= rows of a tile)
+constexpr int kThreads = 1024;     // 16 waves
+constexpr int kRowStride = 20;     // floats per panel row in LDS (80 B: rows start on varied banks)
+constexpr size_t kLds = (size_t)kBR * kRowStride * sizeof(float) + (size_t)kBC * 16 * sizeof(float);  // 144 KB
+static_assert(kBC <= kThreads && kBC < 2048 && kBR == 1024, "slot and gather fields");
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -50,6 +51,7 @@ struct Layout {
   const uint32_t* ent;
   const uint32_t* val;
   const int64_t* tile_start;  // [n_blocks * n_panels + 1]
+  const uint16_t* tile_off;   // [n_blocks * n_panels][16] start of row i + 1 of each tile
   const int32_t* row0;        // [n_blocks + 1] first out position of each block
   const int32_t* out_row;     // [n_out] original row of each out position
   const int32_t* vslot;       // [n_out] first slot of each out position, block-relative
@@ -58,19 +60,9 @@ struct Layout {
   int32_t packed, value_type;
 };
 
-__device__ __forceinline__ uint32_t quad_bcast(uint32_t v, int j) {
-  // DPP quad_perm [j, j, j, j]: every lane of a 4-lane group takes lane j's value
-  switch (j) {
-    case 0: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x00, 0xF, 0xF, false);
-    case 1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x55, 0xF, 0xF, false);
-    case 2: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xAA, 0xF, 0xF, false);
-    default: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xFF, 0xF, 0xF, false);
-  }
-}
-
 template <bool PACK, bool FV>
 __device__ __forceinline__ float entry_value(uint32_t e, uint32_t v) {
-  if (PACK) return (float)(e >> 22);
+  if (PACK) return (float)((e >> 10) & 0x3FF);
   return FV ? __uint_as_float(v) : (float)v;
 }
 
@@ -91,19 +83,17 @@ struct Epi {
 template <bool PACK, bool FV, int EPI>
 __global__ void __launch_bounds__(kThreads) product_kernel(Layout L, const f4* __restrict__ dense, Epi ep) {
   extern __shared__ f4 smem[];
-  f4* panel = smem;              // kBR rows x 4 float4
-  f4* acc_l = smem + kBR * 4;    // kBC slots x 4 float4
+  float* panel = reinterpret_cast<float*>(smem);                 // kBR rows x kRowStride floats
+  f4* acc_l = smem + (size_t)kBR * kRowStride / 4;                // kBC slots x 4 float4
   const int64_t blk = blockIdx.x;
   const int tid = threadIdx.x;
-  const int q = tid & 3;
-  const int g = tid >> 2;
   const int r0 = L.row0[blk], r1 = L.row0[blk + 1];
   const int nslots = L.nslots[blk];
   const f4 zero = {0.f, 0.f, 0.f, 0.f};
   for (int i = tid; i < nslots * 4; i += kThreads) acc_l[i] = zero;
   const int64_t P = L.n_panels;
   // panel prefetch: float4 number tid + k * kThreads of the panel, k = 0..3
-  // (rows (tid >> 2) + 256 k, quarter tid & 3)
+  // (row (tid >> 2) + 256 k, quarter tid & 3)
   const int64_t prow = tid >> 2;
   const int pq = tid & 3;
 #define SG_PANEL_LOAD(P0)                                                                  \
@@ -114,73 +104,75 @@ __global__ void __launch_bounds__(kThreads) product_kernel(Layout L, const f4* _
     pre2 = rb + 512 < L.n_gather ? dense[(rb + 512) * 4 + pq] : zero;                      \
     pre3 = rb + 768 < L.n_gather ? dense[(rb + 768) * 4 + pq] : zero;                      \
   }
+  // Thread tid's piece of tile P0: its length LEN (rows i whose count of pieces
+  // exceeds tid) and its entries EB[i] = ent[start + off_i + tid], i < LEN.
+#define SG_PIECE_LOAD(P0, LEN, EB, VB)                                                     \
+  {                                                                                        \
+    const int64_t t_ = blk * P + (P0);                                                     \
+    const int64_t st_ = L.tile_start[t_];                                                  \
+    const uint16_t* of_ = L.tile_off + t_ * 16;                                            \
+    int prev_ = 0;                                                                         \
+    LEN = 0;                                                                               \
+    _Pragma("unroll") for (int i = 0; i < kPiece; ++i) {                                   \
+      const int o_ = of_[i];                                                               \
+      const bool in_ = tid < o_ - prev_;                                                   \
+      LEN += in_ ? 1 : 0;                                                                  \
+      const int64_t at_ = st_ + prev_ + tid;                                               \
+      EB[i] = (in_ && at_ < L.n_entries) ? __builtin_nontemporal_load(L.ent + at_) : 0u;   \
+      if (!PACK) VB[i] = (in_ && at_ < L.n_entries) ? __builtin_nontemporal_load(L.val + at_) : 0u; \
+      prev_ = o_;                                                                          \
+    }                                                                                      \
+  }
   f4 pre0, pre1, pre2, pre3;
+  uint32_t ecur[kPiece], vcur[kPiece], enxt[kPiece], vnxt[kPiece];
+  int len_cur = 0, len_nxt = 0;
+  SG_PIECE_LOAD(0, len_cur, ecur, vcur)
   SG_PANEL_LOAD(0)
-  const int64_t* ts = L.tile_start + blk * P;
   for (int64_t p = 0; p < P; ++p) {
     __syncthreads();
-    panel[tid] = pre0;
-    panel[tid + kThreads] = pre1;
-    panel[tid + 2 * kThreads] = pre2;
-    panel[tid + 3 * kThreads] = pre3;
-    __syncthreads();
-    const int64_t t0 = min(ts[p], L.n_entries), t1 = min(ts[p + 1], L.n_entries);
-    const int64_t len = t1 - t0;
-    const int64_t a = t0 + (len * g) / kGroups;
-    const int64_t e_end = t0 + (len * (g + 1)) / kGroups;
-    // the group's first four entries are requested before the next panel, so
-    // waiting for them does not wait for the panel's loads
-    uint32_t e_nxt = a + q < t1 ? __builtin_nontemporal_load(L.ent + a + q) : 0u;
-    uint32_t v_nxt = (!PACK && a + q < t1) ? __builtin_nontemporal_load(L.val + a + q) : 0u;
-    if (p + 1 < P) SG_PANEL_LOAD(p + 1)
-    if (len == 0) continue;
-    bool active = false;
-    int slot = 0;
-    f4 acc = zero;
-    for (int64_t pos = a; pos < t1; pos += 4) {
-      const uint32_t e_own = e_nxt, v_own = v_nxt;
-      {  // the next four, one step ahead
-        const int64_t nx = pos + 4 + q;
-        e_nxt = nx < t1 ? __builtin_nontemporal_load(L.ent + nx) : 0u;
-        v_nxt = (!PACK && nx < t1) ? __builtin_nontemporal_load(L.val + nx) : 0u;
-      }
-      bool stop = false;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t ej = quad_bcast(e_own, j);
-        const uint32_t vj = PACK ? 0u : quad_bcast(v_own, j);
-        if (pos + j >= t1) {
-          stop = true;
-          break;
-        }
-        if (ej & kFirst) {
-          if (active) {
-            acc_l[slot * 4 + q] += acc;
-          }
-          if (pos + j >= e_end) {
-            active = false;
-            stop = true;
-            break;
-          }
-          slot = (int)((ej >> 10) & 0x7FF);
-          active = slot < nslots;  // always, for a layout from snapgpu_block_build
-          acc = zero;
-        }
-        if (active) {
-          const f4 d = panel[(ej & 0x3FF) * 4 + q];
-          const float c = entry_value<PACK, FV>(ej, vj);
-          acc.x = fmaf(c, d.x, acc.x);
-          acc.y = fmaf(c, d.y, acc.y);
-          acc.z = fmaf(c, d.z, acc.z);
-          acc.w = fmaf(c, d.w, acc.w);
-        }
-      }
-      if (stop) break;
+    {
+      float* rowp = panel + prow * kRowStride + pq * 4;
+      *reinterpret_cast<f4*>(rowp) = pre0;
+      *reinterpret_cast<f4*>(rowp + 256 * kRowStride) = pre1;
+      *reinterpret_cast<f4*>(rowp + 512 * kRowStride) = pre2;
+      *reinterpret_cast<f4*>(rowp + 768 * kRowStride) = pre3;
     }
-    if (active) {
-      acc_l[slot * 4 + q] += acc;
+    __syncthreads();
+    // the next tile's piece and panel are in flight while this tile runs
+    if (p + 1 < P) {
+      SG_PIECE_LOAD(p + 1, len_nxt, enxt, vnxt)
+      SG_PANEL_LOAD(p + 1)
+    }
+    if (len_cur > 0) {
+      f4 a0 = zero, a1 = zero, a2 = zero, a3 = zero;
+#pragma unroll
+      for (int i = 0; i < kPiece; ++i) {
+        if (i < len_cur) {
+          const uint32_t e = ecur[i];
+          const float c = entry_value<PACK, FV>(e, vcur[i]);
+          const float* rp = panel + (e & 0x3FF) * kRowStride;
+          a0 += c * *reinterpret_cast<const f4*>(rp);
+          a1 += c * *reinterpret_cast<const f4*>(rp + 4);
+          a2 += c * *reinterpret_cast<const f4*>(rp + 8);
+          a3 += c * *reinterpret_cast<const f4*>(rp + 12);
+        }
+      }
+      const int slot = (int)((ecur[0] >> 20) & 0x7FF);
+      if (slot < nslots) {  // always, for a layout from snapgpu_block_build
+        acc_l[slot * 4] += a0;
+        acc_l[slot * 4 + 1] += a1;
+        acc_l[slot * 4 + 2] += a2;
+        acc_l[slot * 4 + 3] += a3;
+      }
+    }
+    len_cur = len_nxt;
+#pragma unroll
+    for (int i = 0; i < kPiece; ++i) {
+      ecur[i] = enxt[i];
+      vcur[i] = vnxt[i];
     }
   }
+#undef SG_PIECE_LOAD
 #undef SG_PANEL_LOAD
   __syncthreads();
   // epilogue: row r (out position r0 + i) adds its slots in order
@@ -296,11 +288,20 @@ __device__ __forceinline__ int64_t table_pos(const PosTables& T, int blk, int64_
 // FILL = false: the last entry of each (row, panel) run writes the lengths of
 // the run's pieces into Lq (zeroed beforehand). FILL = true: every entry goes
 // to E[its piece] + its offset in the piece.
+// FILL: entry i of a piece goes to row i of its tile, at the piece's rank:
+// tile start + start of row i + rank.
+struct FillTables {
+  const int64_t* E;          // exclusive scan of the piece lengths (tile starts at slot 0)
+  const int32_t* rank;       // rank of each piece within its tile (same indexing as Lq)
+  const uint16_t* tile_off;  // [tile][16]: start of row i + 1
+  int64_t P;
+};
+
 template <bool FILL, bool PACK>
 __global__ void __launch_bounds__(256) pieces_kernel(const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
                                                      const uint32_t* __restrict__ val, int64_t n, PosTables T,
                                                      int64_t table_size, int64_t nnz, int64_t* __restrict__ Lq,
-                                                     const int64_t* __restrict__ E, uint32_t* __restrict__ ent,
+                                                     FillTables F, uint32_t* __restrict__ ent,
                                                      uint32_t* __restrict__ val_out, int32_t* __restrict__ err) {
   const int lane = threadIdx.x & 63;
   const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -330,14 +331,20 @@ __global__ void __launch_bounds__(256) pieces_kernel(const int64_t* __restrict__
         }
       } else {
         const int piece = (int)(i / kPiece);
+        const int row = (int)(i % kPiece);
         const int64_t tp = table_pos(T, blk, pnl, vs + piece);
-        const int64_t dst = (tp >= 0 && tp < table_size) ? E[tp] + (i % kPiece) : -1;
+        const int64_t tile = (int64_t)blk * F.P + pnl;
+        int64_t dst = -1;
+        if (tp >= 0 && tp < table_size) {
+          const int64_t tstart = F.E[table_pos(T, blk, pnl, 0)];
+          const int64_t roff = row == 0 ? 0 : (int64_t)F.tile_off[tile * 16 + row - 1];
+          dst = tstart + roff + F.rank[tp];
+        }
         if (dst < 0 || dst >= nnz) atomicOr(err, 2);
-        const uint32_t lo = (uint32_t)(col - pnl * kBR) | ((uint32_t)(vs + piece) << 10) |
-                            (i % kPiece == 0 ? kFirst : 0u);
+        const uint32_t lo = (uint32_t)(col - pnl * kBR) | ((uint32_t)(vs + piece) << 20);
         if (dst >= 0 && dst < nnz) {
           if (PACK) {
-            ent[dst] = lo | (val[e] << 22);
+            ent[dst] = lo | (val[e] << 10);
           } else {
             ent[dst] = lo;
             val_out[dst] = val[e];
@@ -346,6 +353,60 @@ __global__ void __launch_bounds__(256) pieces_kernel(const int64_t* __restrict__
       }
     }
     rs.advance(b, starts);
+  }
+}
+
+// One wave per tile (block b, panel p): the pieces (slots with a length > 0)
+// ordered by decreasing length, then slot; rank[slot] = position in that order,
+// tile_off[tile][i] = start of row i + 1 (row i holds entry i of every piece
+// longer than i, so it has c_i = #pieces longer than i entries).
+__global__ void __launch_bounds__(256) tile_order_kernel(const int64_t* __restrict__ Lq, const int64_t* __restrict__ base,
+                                                         const int32_t* __restrict__ nslots, int64_t n_blocks, int64_t P,
+                                                         int32_t* __restrict__ rank, uint16_t* __restrict__ tile_off) {
+  const int lane = threadIdx.x & 63;
+  const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (t >= n_blocks * P) return;
+  const int64_t b = t / P, p = t - b * P;
+  const int ns = nslots[b];
+  const int64_t t0 = base[b] + p * ns;
+  // counts of pieces per length 1..kPiece (wave-uniform)
+  int cnt[kPiece + 1];
+#pragma unroll
+  for (int v = 0; v <= kPiece; ++v) cnt[v] = 0;
+  for (int s0 = 0; s0 < ns; s0 += 64) {
+    const int s = s0 + lane;
+    const int len = s < ns ? (int)Lq[t0 + s] : 0;
+#pragma unroll
+    for (int v = 1; v <= kPiece; ++v) cnt[v] += popc64(ballot64(len == v));
+  }
+  // c_i = #pieces longer than i; the pieces of length v start at rank c_v
+  int c[kPiece + 1];
+  c[kPiece] = 0;
+#pragma unroll
+  for (int v = kPiece - 1; v >= 0; --v) c[v] = c[v + 1] + cnt[v + 1];
+  if (lane == 0) {
+    int off = 0;
+#pragma unroll
+    for (int i = 0; i < kPiece; ++i) {
+      off += c[i];
+      tile_off[t * 16 + i] = (uint16_t)off;
+    }
+  }
+  int run[kPiece + 1];
+#pragma unroll
+  for (int v = 0; v <= kPiece; ++v) run[v] = 0;
+  const uint64_t lt = lanemask_lt(lane);
+  for (int s0 = 0; s0 < ns; s0 += 64) {
+    const int s = s0 + lane;
+    const int len = s < ns ? (int)Lq[t0 + s] : 0;
+    int r = 0;
+#pragma unroll
+    for (int v = 1; v <= kPiece; ++v) {
+      const uint64_t m = ballot64(len == v);
+      if (len == v) r = c[v] + run[v] + popc64(m & lt);
+      run[v] += popc64(m);
+    }
+    if (s < ns) rank[t0 + s] = len > 0 ? r : -1;
   }
 }
 
@@ -517,14 +578,14 @@ int snapgpu_block_plan(const int64_t* d_ptr, const int32_t* d_idx, const uint32_
 
 int64_t snapgpu_block_build_workspace_bytes(int64_t table_size) {
   const int64_t t = std::max<int64_t>(table_size, 0) + 1;
-  return 2 * align256(t * 8) + align256(snapgpu_scan_workspace_bytes(t)) + 768;
+  return 2 * align256(t * 8) + align256(t * 4) + align256(snapgpu_scan_workspace_bytes(t)) + 768;
 }
 
 int snapgpu_block_build(const int64_t* d_ptr, const int32_t* d_idx, const uint32_t* d_val, int64_t nrows,
                         int64_t dense_rows, int64_t nnz, int32_t packed, const void* d_plan_ws, int64_t plan_ws_bytes,
                         int64_t n_blocks, int64_t table_size, void* d_ws, int64_t ws_bytes, uint32_t* d_ent,
-                        uint32_t* d_val_out, int64_t* d_tile_start, int32_t* d_row0, int32_t* d_out_row,
-                        int32_t* d_vslot, int32_t* d_nslots, void* stream) {
+                        uint32_t* d_val_out, int64_t* d_tile_start, uint16_t* d_tile_off, int32_t* d_row0,
+                        int32_t* d_out_row, int32_t* d_vslot, int32_t* d_nslots, void* stream) {
   clear_error();
   SG_ARG(plan_ws_bytes >= snapgpu_block_plan_workspace_bytes(nrows), "plan workspace too small");
   SG_ARG(ws_bytes >= snapgpu_block_build_workspace_bytes(table_size), "workspace too small");
@@ -539,6 +600,8 @@ int snapgpu_block_build(const int64_t* d_ptr, const int32_t* d_idx, const uint32
   q += align256(t * 8);
   int64_t* E = (int64_t*)q;
   q += align256(t * 8);
+  int32_t* R = (int32_t*)q;
+  q += align256(t * 4);
   int32_t* err = (int32_t*)q;
   q += 256;
   void* scan_ws = q;
@@ -547,18 +610,23 @@ int snapgpu_block_build(const int64_t* d_ptr, const int32_t* d_idx, const uint32
   SG_HIP(hipMemsetAsync(Lq, 0, t * 8, st));
   SG_HIP(hipMemsetAsync(err, 0, 4, st));
   const unsigned gw = (unsigned)ceil_div(nrows * 64, 256);
-  pieces_kernel<false, true><<<gw, 256, 0, st>>>(d_ptr, d_idx, d_val, nrows, T, table_size, nnz, Lq, nullptr,
-                                                 nullptr, nullptr, err);
+  const FillTables F{E, R, d_tile_off, P};
+  pieces_kernel<false, true><<<gw, 256, 0, st>>>(d_ptr, d_idx, d_val, nrows, T, table_size, nnz, Lq, F, nullptr,
+                                                 nullptr, err);
+  SG_LAUNCH_CHECK();
+  if (n_blocks * P > 0)
+    tile_order_kernel<<<(unsigned)ceil_div(n_blocks * P * 64, 256), 256, 0, st>>>(Lq, w.base, w.nslots, n_blocks, P,
+                                                                                R, d_tile_off);
   SG_LAUNCH_CHECK();
   int rc = snapgpu_scan_inclusive_i64(Lq, t, scan_ws, scan_b, stream);
   if (rc) return rc;
   shift_kernel<<<(unsigned)ceil_div(t + 1, 256), 256, 0, st>>>(Lq, table_size, E);
   SG_LAUNCH_CHECK();
   if (packed)
-    pieces_kernel<true, true><<<gw, 256, 0, st>>>(d_ptr, d_idx, d_val, nrows, T, table_size, nnz, nullptr, E, d_ent,
+    pieces_kernel<true, true><<<gw, 256, 0, st>>>(d_ptr, d_idx, d_val, nrows, T, table_size, nnz, nullptr, F, d_ent,
                                                   nullptr, err);
   else
-    pieces_kernel<true, false><<<gw, 256, 0, st>>>(d_ptr, d_idx, d_val, nrows, T, table_size, nnz, nullptr, E,
+    pieces_kernel<true, false><<<gw, 256, 0, st>>>(d_ptr, d_idx, d_val, nrows, T, table_size, nnz, nullptr, F,
                                                    d_ent, d_val_out, err);
   tile_start_kernel<<<(unsigned)ceil_div(n_blocks * P + 1, 256), 256, 0, st>>>(E, w.base, w.nslots, n_blocks, P, nnz,
                                                                             d_tile_start);
@@ -589,6 +657,7 @@ Layout to_layout(const snapgpu_block_layout& B) {
   L.ent = B.ent;
   L.val = B.val;
   L.tile_start = B.tile_start;
+  L.tile_off = B.tile_off;
   L.row0 = B.row0;
   L.out_row = B.out_row;
   L.vslot = B.vslot;

@@ -198,6 +198,7 @@ class SpectralOperator:
         ent = torch.empty(max(nnz, 1), **i32)
         val_out = None if packed.value else torch.empty(max(nnz, 1), **i32)
         tile_start = torch.empty(nb.value * P + 1, dtype=torch.int64, device=dev)
+        tile_off = torch.empty(max(nb.value * P, 1) * 16, dtype=torch.int16, device=dev)
         row0 = torch.empty(nb.value + 1, **i32)
         out_row = torch.empty(nrows, **i32)
         vslot = torch.empty(nrows, **i32)
@@ -207,10 +208,12 @@ class SpectralOperator:
         _lib.check(L.snapgpu_block_build(
             ptr.data_ptr(), idx.data_ptr(), val.data_ptr(), nrows, dense_rows, nnz, packed.value, pws.data_ptr(),
             pws_b, nb.value, ts.value, bws.data_ptr(), bws_b, ent.data_ptr(), _lib.ptr(val_out),
-            tile_start.data_ptr(), row0.data_ptr(), out_row.data_ptr(), vslot.data_ptr(), nslots.data_ptr(), st))
+            tile_start.data_ptr(), tile_off.data_ptr(), row0.data_ptr(), out_row.data_ptr(), vslot.data_ptr(),
+            nslots.data_ptr(), st))
         del pws, bws
-        self._panel_keep.append((ent, val_out, tile_start, row0, out_row, vslot, nslots))
-        return _lib.BlockLayout(ent.data_ptr(), _lib.ptr(val_out), tile_start.data_ptr(), row0.data_ptr(),
+        self._panel_keep.append((ent, val_out, tile_start, row0, out_row, vslot, nslots, tile_off))
+        return _lib.BlockLayout(ent.data_ptr(), _lib.ptr(val_out), tile_start.data_ptr(), tile_off.data_ptr(),
+                                row0.data_ptr(),
                                 out_row.data_ptr(), vslot.data_ptr(), nslots.data_ptr(), nb.value, P, nrows,
                                 dense_rows, nnz, packed.value, self.value_type)
 

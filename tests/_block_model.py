@@ -9,8 +9,7 @@ before any kernel runs."""
 import numpy as np
 import scipy.sparse as sp
 
-kBR, kBC, kPiece, kGroups = 1024, 1280, 16, 256
-kFirst = 1 << 21
+kBR, kBC, kPiece = 1024, 1024, 16
 
 def runscan_rows(idx_row):
     """emulates RunScan over one row: returns per entry (s, last)"""
@@ -68,55 +67,68 @@ def build(X, dense_rows, target_blocks):
                 Lq[tp] = min(kPiece, ln - j * kPiece)
     incl = np.cumsum(Lq); E = np.concatenate([[0], incl[:-1]])
     assert E[ts] == nnz, (E[ts], nnz)
+    # per tile: pieces by decreasing length, then slot; row i holds entry i of
+    # every piece longer than i
+    rank = np.full(ts + 1, -1, np.int64)
+    tile_off = np.zeros((nb * P, kPiece), np.int64)
+    for blk in range(nb):
+        for p in range(P):
+            t0 = tpos(blk, p, 0); lens_t = Lq[t0:t0 + nslots[blk]]
+            slots = np.flatnonzero(lens_t > 0)
+            order_t = sorted(slots, key=lambda s_: (-lens_t[s_], s_))
+            for r_, s_ in enumerate(order_t):
+                rank[t0 + s_] = r_
+            c = np.array([(lens_t > i).sum() for i in range(kPiece)])
+            tile_off[blk * P + p] = np.cumsum(c)
+    assert tile_off.max(initial=0) < 65536
     ent = np.full(nnz, -1, np.int64); val = np.zeros(nnz, np.int64)
     for q, r in enumerate(order):
         a = X.indptr[r]; s_arr, last = runinfo[r]
         for k in range(X.indptr[r+1] - a):
-            col = X.indices[a+k]; pnl = col // kBR; i = k - s_arr[k]; piece = i // kPiece
-            tp = tpos(qblock[q], pnl, vslot[q] + piece); dst = E[tp] + i % kPiece
-            assert 0 <= dst < nnz and ent[dst] == -1
+            col = X.indices[a+k]; pnl = col // kBR; i = k - s_arr[k]; piece = i // kPiece; row = i % kPiece
             slot = vslot[q] + piece; assert slot < nslots[qblock[q]]
-            ent[dst] = (col - pnl * kBR) | (slot << 10) | (kFirst if i % kPiece == 0 else 0)
+            tp = tpos(qblock[q], pnl, slot)
+            tile = qblock[q] * P + pnl
+            dst = E[tpos(qblock[q], pnl, 0)] + (0 if row == 0 else tile_off[tile, row - 1]) + rank[tp]
+            assert 0 <= dst < nnz and ent[dst] == -1
+            ent[dst] = (col - pnl * kBR) | (slot << 20)
             val[dst] = X.data[a+k]
     assert (ent >= 0).all()
     tile_start = np.array([E[base[b] + p * nslots[b]] for b in range(nb) for p in range(P)] + [nnz])
     assert (np.diff(tile_start) >= 0).all()
     return dict(order=order, row0=np.array(row0), vslot=vslot, nslots=np.array(nslots), tile_start=tile_start,
-                ent=ent, val=val, P=P, nb=nb)
+                tile_off=tile_off, ent=ent, val=val, P=P, nb=nb)
 
 def product(Lay, D, n_out):
+    """Thread k of a block's workgroup owns the k-th piece of each tile: its
+    entries are at tile start + start of row i + k for the rows i whose piece
+    count exceeds k; each piece's sum goes to its slot."""
     out = np.zeros((n_out, D.shape[1]))
     P = Lay['P']
     for b in range(Lay['nb']):
         r0, r1 = Lay['row0'][b], Lay['row0'][b+1]; ns = Lay['nslots'][b]
-        acc_l = np.zeros((max(ns,1), D.shape[1]))
+        acc_l = np.zeros((max(ns, 1), D.shape[1]))
         for p in range(P):
-            t0, t1 = Lay['tile_start'][b*P + p], Lay['tile_start'][b*P + p + 1]
-            L = t1 - t0
-            if L == 0: continue
-            owned = np.zeros(L, int)
-            for g in range(kGroups):
-                a = t0 + (L * g) // kGroups; e_end = t0 + (L * (g+1)) // kGroups
-                active = False; slot = 0; acc = 0
-                pos = a; stop = False
-                while pos < t1 and not stop:
-                    for j in range(4):
-                        if pos + j >= t1: stop = True; break
-                        ej = Lay['ent'][pos + j]
-                        if ej & kFirst:
-                            if active: acc_l[slot] += acc
-                            if pos + j >= e_end: active = False; stop = True; break
-                            slot = (ej >> 10) & 0x7FF; active = slot < ns; assert active
-                            acc = 0
-                        if active:
-                            gl = ej & 0x3FF; assert gl < kBR
-                            acc = acc + Lay['val'][pos + j] * D[p * kBR + gl]
-                            owned[pos + j - t0] += 1
-                    pos += 4
-                if active: acc_l[slot] += acc
-            assert (owned == 1).all(), "every entry processed exactly once"
+            t = b * P + p
+            st = Lay['tile_start'][t]; offs = Lay['tile_off'][t]
+            seen = 0
+            for k in range(kBC):
+                prev = 0; acc = 0; slot = None
+                for i in range(kPiece):
+                    if k < offs[i] - prev:
+                        at = st + prev + k
+                        assert at < Lay['tile_start'][t + 1]
+                        e = Lay['ent'][at]; gl = e & 0x3FF; s_ = (e >> 20) & 0x7FF
+                        assert slot is None or s_ == slot
+                        slot = s_
+                        acc = acc + Lay['val'][at] * D[p * kBR + gl]
+                        seen += 1
+                    prev = offs[i]
+                if slot is not None:
+                    assert slot < ns
+                    acc_l[slot] += acc
+            assert seen == Lay['tile_start'][t + 1] - st, "every entry of the tile read once"
         for i, r in enumerate(range(r0, r1)):
             s0 = Lay['vslot'][r]; s1 = Lay['vslot'][r+1] if r + 1 < r1 else ns
             out[Lay['order'][r]] = acc_l[s0:s1].sum(0)
     return out
-

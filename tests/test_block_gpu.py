@@ -37,14 +37,17 @@ def test_layout_arrays_equal_the_model(torch_gpu):
     op = SpectralOperator(_to_device_csr(X, torch), layout="block")
     L = op.block_csr
     mod = bm.build(X, X.shape[1], 512)
-    keep = dict(zip(("ent", "val", "tile_start", "row0", "out_row", "vslot", "nslots"), op._panel_keep[0]))
+    keep = dict(zip(("ent", "val", "tile_start", "row0", "out_row", "vslot", "nslots", "tile_off"),
+                    op._panel_keep[0]))
     assert L.n_blocks == mod["nb"] and L.n_panels == mod["P"] and L.packed == 1
     assert np.array_equal(keep["out_row"].cpu().numpy(), mod["order"])
     assert np.array_equal(keep["row0"].cpu().numpy(), mod["row0"])
     assert np.array_equal(keep["vslot"].cpu().numpy(), mod["vslot"])
     assert np.array_equal(keep["nslots"].cpu().numpy()[:mod["nb"]], mod["nslots"])
     assert np.array_equal(keep["tile_start"].cpu().numpy(), mod["tile_start"])
-    packed = mod["ent"] | (mod["val"] << 22)
+    assert np.array_equal(keep["tile_off"].cpu().numpy().view(np.uint16).reshape(-1, 16)[:mod["nb"] * mod["P"]],
+                          mod["tile_off"])
+    packed = mod["ent"] | (mod["val"] << 10)
     assert np.array_equal(keep["ent"].cpu().numpy().view(np.uint32)[:X.nnz], packed.astype(np.uint32))
 
 
