@@ -287,19 +287,23 @@ def knn_precision(torch, n, k=50, d=30):
             "identical_graphs": bool(same)}
 
 
-def embedding_quality(torch, pinfo, g_learned, g_plain, cfg, steps):
+def embedding_quality(torch, pinfo, emb_learned, g_learned, emb_plain, g_plain, cfg, steps):
     """Learned against IDF feature weights, judged on what the generator knows:
     each cell's type. kNN purity (the fraction of a cell's k neighbours of its
-    own type) of the timed pipeline's graph (weights after one ascent step), of
-    the plain path's (IDF), and, when steps > 0, of a graph after `steps` ascent
-    steps from IDF (untimed), with that run's eigengap history (FP64 gaps)."""
+    own type) and type separation (type_separation) of the timed pipeline's
+    embedding (weights after one ascent step), of the plain path's (IDF), and,
+    when steps > 0, of an embedding after `steps` ascent steps from IDF
+    (untimed), with that run's eigengap history (FP64 gaps)."""
     from snapatac2_amd import synth
     from snapatac2_amd.pp._knn import nearest_neighbour_graph
     from snapatac2_amd.tl._adjoint import tune_feature_weights
     n, k = cfg[0], cfg[3]
-    types = synth.cell_types(n).cpu().numpy()
+    types_t = synth.cell_types(n)
+    types = types_t.cpu().numpy()
     out = {"k": k, "purity_idf": round(knn_purity(g_plain, types, k), 5),
-           "purity_learned_1_step": round(knn_purity(g_learned, types, k), 5)}
+           "purity_learned_1_step": round(knn_purity(g_learned, types, k), 5),
+           "separation_idf": type_separation(torch, emb_plain, types_t),
+           "separation_learned_1_step": type_separation(torch, emb_learned, types_t)}
     if steps > 0:
         op = pinfo["op"]
         t0 = time.perf_counter()
@@ -307,9 +311,25 @@ def embedding_quality(torch, pinfo, g_learned, g_plain, cfg, steps):
         emb, _ = _embed(torch, tr.evals, tr.vectors, pinfo["n_comps"])
         g = nearest_neighbour_graph(emb, k, precision=cfg[4])
         out[f"purity_learned_{steps}_steps"] = round(knn_purity(g, types, k), 5)
+        out[f"separation_learned_{steps}_steps"] = type_separation(torch, emb, types_t)
         out[f"gaps_{steps}_steps"] = [float(h[0]) for h in tr.history]
         out[f"ascent_{steps}_steps_s"] = round(time.perf_counter() - t0, 2)
     return out
+
+
+def type_separation(torch, E, types):
+    """Scale-free separation of the generator's cell types in an embedding E
+    (n x d): the mean over cells of (distance to the nearest other type's
+    centroid) / (distance to the own type's centroid). kNN purity saturates at
+    1 when the types are well apart; this ratio keeps growing with the margin."""
+    t = types.to(E.device).long()
+    nt = int(t.max().item()) + 1
+    cnt = torch.bincount(t, minlength=nt).to(E.dtype)
+    mu = torch.zeros((nt, E.shape[1]), dtype=E.dtype, device=E.device).index_add_(0, t, E) / cnt[:, None]
+    d = torch.cdist(E, mu)
+    own = d.gather(1, t[:, None]).squeeze(1)
+    d.scatter_(1, t[:, None], float("inf"))
+    return round(float((d.min(1).values / own.clamp_min(1e-300)).mean().item()), 5)
 
 
 def knn_purity(g, types, k):
@@ -799,7 +819,7 @@ def main():
         del info["op"]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        pinfo, _, g_plain = run_pipeline(torch, chunks, cfg, ptimer, None, learn_steps=0)
+        pinfo, emb_plain, g_plain = run_pipeline(torch, chunks, cfg, ptimer, None, learn_steps=0)
         torch.cuda.synchronize()
         pdt = time.perf_counter() - t0
         nys_X = pinfo["op"].X  # the selected matrix, for the Nystrom measurement
@@ -807,7 +827,8 @@ def main():
                  "stages_s": {k: round(v, 3) for k, v in ptimer.t.items()},
                  "lanczos": pinfo["lanczos"]}
         log(f"[bench] plain path (IDF weights): {pdt:.3f}s")
-        quality = embedding_quality(torch, pinfo, g, g_plain, cfg, args.ascent_steps)
+        quality = embedding_quality(torch, pinfo, emb, g, emb_plain, g_plain, cfg, args.ascent_steps)
+        del emb_plain
         log(f"[bench] embedding quality: {quality}")
         del pinfo["op"]
     log(f"[bench] lanczos {info['lanczos']}  nnz tile={info['tile_nnz']} sel={info['selected_nnz']}")

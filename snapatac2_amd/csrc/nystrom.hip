@@ -74,29 +74,36 @@ __global__ void __launch_bounds__(kThreads) row_inv_norm_kernel(const int64_t* _
 }
 
 // C[i, :] = a_i * sum_e cnt_e * b[idx_e] * B[idx_e, :] over the entries of row i.
-// One wave per row: the two half-waves take alternate entries, lane % 32 is
-// the column inside a 32-column block; the halves are added at the end.
+// One wave per row: lane % KP is the column inside a KP-column block and the
+// 64 / KP lane groups take every (64 / KP)-th entry; the groups' sums are added
+// by a butterfly over the group bits. KP is the smallest power of two >= k (at
+// most 32), so a narrow block keeps every lane busy; the summation order is
+// fixed for a given k.
+template <int KP>
 __global__ void __launch_bounds__(kThreads) spmm_scaled_kernel(
     const int64_t* __restrict__ indptr, const int32_t* __restrict__ idx, const uint32_t* __restrict__ cnt,
     int64_t n, const double* __restrict__ a, const double* __restrict__ b, const double* __restrict__ B,
     int64_t ldb, int k, double* __restrict__ C, int64_t ldc) {
+  constexpr int G = 64 / KP;
   const int lane = threadIdx.x & 63;
-  const int half = lane >> 5, col = lane & 31;
+  const int grp = lane / KP, col = lane % KP;
   const int64_t row = ((int64_t)blockIdx.x * kThreads + threadIdx.x) >> 6;
   if (row >= n) return;
   const int64_t e0 = indptr[row], e1 = indptr[row + 1];
   const double ai = a ? a[row] : 1.0;
-  for (int c0 = 0; c0 < k; c0 += 32) {
+  for (int c0 = 0; c0 < k; c0 += KP) {
     const int c = c0 + col;
     const bool on = c < k;
     double acc = 0.0;
-    for (int64_t e = e0 + half; e < e1; e += 2) {
+#pragma unroll 4
+    for (int64_t e = e0 + grp; e < e1; e += G) {
       const int32_t j = idx[e];
       const double x = (double)cnt[e] * (b ? b[j] : 1.0);
       acc = fma(x, on ? B[(int64_t)j * ldb + c] : 0.0, acc);
     }
-    acc += __shfl_xor(acc, 32, 64);
-    if (half == 0 && on) C[row * ldc + c] = ai * acc;
+#pragma unroll
+    for (int off = KP; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
+    if (grp == 0 && on) C[row * ldc + c] = ai * acc;
   }
 }
 
@@ -261,8 +268,17 @@ int snapgpu_csr_spmm_scaled_f64(const int64_t* d_indptr, const int32_t* d_idx, c
   clear_error();
   SG_ARG(n >= 0 && k >= 1 && ldb >= k && ldc >= k && d_B != nullptr && d_C != nullptr, "bad arguments");
   if (n == 0) return SNAPGPU_OK;
-  spmm_scaled_kernel<<<waves_grid(n), kThreads, 0, as_stream(stream)>>>(d_indptr, d_idx, d_cnt, n, d_row_scale,
-                                                                        d_col_scale, d_B, ldb, k, d_C, ldc);
+  hipStream_t st = as_stream(stream);
+#define SG_SPMM(KP)                                                                                            \
+  spmm_scaled_kernel<KP><<<waves_grid(n), kThreads, 0, st>>>(d_indptr, d_idx, d_cnt, n, d_row_scale, d_col_scale, \
+                                                             d_B, ldb, k, d_C, ldc)
+  if (k <= 1) SG_SPMM(1);
+  else if (k <= 2) SG_SPMM(2);
+  else if (k <= 4) SG_SPMM(4);
+  else if (k <= 8) SG_SPMM(8);
+  else if (k <= 16) SG_SPMM(16);
+  else SG_SPMM(32);
+#undef SG_SPMM
   SG_LAUNCH_CHECK();
   return SNAPGPU_OK;
 }

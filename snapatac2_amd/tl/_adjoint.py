@@ -103,18 +103,19 @@ def _gap_stats(op: SpectralOperator, n_components: int, seed: int, v0=None):
     return float(abs(evals[k - 1]) - abs(evals[k])), evals, U, st
 
 
-def _checked_gap(op: SpectralOperator, evals, U, st, k: int):
+def _checked_gap(op: SpectralOperator, evals, U, st, k: int, pair=None):
     """(gap, uncertainty) of |lambda_k| - |lambda_{k+1}| for an acceptance test.
     With integer counts the two eigenvalues are re-estimated as FP64 Rayleigh
     quotients of their Lanczos vectors, so the FP32 products' rounding (about
     1e-7 of |lambda_1|, the size of the gap at a million cells) drops out; what
     remains is the Rayleigh quotient's error, at most residual^2 / gap per
-    eigenvalue, plus FP64 summation rounding."""
+    eigenvalue, plus FP64 summation rounding. `pair`: the (lambda_k,
+    lambda_{k+1}) that _pair_f64 already returned for these vectors."""
     res = float(st.get("max_residual", 0.0)) if isinstance(st, dict) else 0.0
     if op.value_type != 0:
         gap = float(abs(evals[k - 1]) - abs(evals[k]))
         return gap, 2.0 * res + 1e-6 * abs(float(evals[0]))
-    _, (l0, l1) = _pair_f64(op, U, evals, k)
+    l0, l1 = pair if pair is not None else _pair_f64(op, U, evals, k)[1]
     gap = abs(l0) - abs(l1)
     return gap, 2.0 * res * res / max(abs(gap), 1e-300) + 1e-12 * abs(float(evals[0]))
 
@@ -122,7 +123,7 @@ def _checked_gap(op: SpectralOperator, evals, U, st, k: int):
 def spectral_gap_grad(op: SpectralOperator, n_components: int = 30, seed: int = 0, v0=None):
     """(gap, dgap/dw, evals, U) by Hellmann-Feynman: d lambda_i = u_i^T dA u_i, i.e. the adjoint
     with V = [u_k, u_{k+1}] and G = [sign(lambda_k) u_k, -sign(lambda_{k+1}) u_{k+1}]."""
-    gap, g, evals, U, _ = _gap_grad_stats(op, n_components, seed, v0)
+    gap, g, evals, U, _, _ = _gap_grad_stats(op, n_components, seed, v0)
     return gap, g, evals, U
 
 
@@ -154,7 +155,7 @@ def _gap_grad_stats(op: SpectralOperator, n_components: int, seed: int, v0=None)
     V, (l0, l1) = _pair_f64(op, U, evals, k)
     sg = torch.tensor([np.sign(l0) or 1.0, -(np.sign(l1) or 1.0)], dtype=torch.float64, device=V.device)
     G = (V * sg).contiguous()
-    return abs(l0) - abs(l1), op.vjp_w(V, G), evals, U, st
+    return abs(l0) - abs(l1), op.vjp_w(V, G), evals, U, st, (l0, l1)
 
 
 def tune_feature_weights(op: SpectralOperator, n_components: int = 30, steps: int = 10,
@@ -175,8 +176,8 @@ def tune_feature_weights(op: SpectralOperator, n_components: int = 30, steps: in
     torch = op.torch
     k = n_components
     w = op.w.clone()
-    _, g, evals, U, st = _gap_grad_stats(op, n_components, seed)
-    gap, unc = _checked_gap(op, evals, U, st, k)
+    _, g, evals, U, st, pair = _gap_grad_stats(op, n_components, seed)
+    gap, unc = _checked_gap(op, evals, U, st, k, pair)
     history = [(gap, 0.0)]
     for it in range(steps):
         if step_size < min_step:
@@ -191,10 +192,10 @@ def tune_feature_weights(op: SpectralOperator, n_components: int = 30, steps: in
             op.set_weights(trial)
             if last:  # no further step: the gradient at the trial point is not needed
                 _, new_evals, new_U, new_st = _gap_stats(op, n_components, seed, v0=U)
-                new_g = None
+                new_g, new_pair = None, None
             else:
-                _, new_g, new_evals, new_U, new_st = _gap_grad_stats(op, n_components, seed, v0=U)
-            new_gap, new_unc = _checked_gap(op, new_evals, new_U, new_st, k)
+                _, new_g, new_evals, new_U, new_st, new_pair = _gap_grad_stats(op, n_components, seed, v0=U)
+            new_gap, new_unc = _checked_gap(op, new_evals, new_U, new_st, k, new_pair)
         except _lib.PanicException:
             new_gap, new_unc = -np.inf, 0.0
         if new_gap - new_unc > gap + unc:

@@ -57,6 +57,23 @@ def test_vjp_matches_oracle(torch_gpu, b):
     assert np.abs(dV - dV_ref).max() < 1e-5 * np.abs(dV_ref).max()
 
 
+@pytest.mark.parametrize("b", [1, 2, 3, 5, 16, 30, 40])
+def test_apply_f64_matches_oracle(torch_gpu, b):
+    """The operator with both sparse products in FP64 (the eigengap acceptance
+    check): against the FP64 oracle to summation rounding, for every column
+    width of the product kernel (1, 2, 4, 8, 16 and 32-column lane groups)."""
+    torch = torch_gpu
+    X = _counts(500, 350, 0.04, seed=20 + b)
+    w = _weights(X, b)
+    V = np.random.default_rng(200 + b).standard_normal((500, b))
+    op = _op(torch, X, w)
+    got = op.apply_f64(torch.from_numpy(V).cuda()).cpu().numpy()
+    ref = adj.apply(X, w, V)
+    assert _rel(got, ref) < 1e-12
+    rq = op.rayleigh_f64(torch.from_numpy(V).cuda()).cpu().numpy()
+    assert np.allclose(rq, (V * ref).sum(0), rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+
+
 def test_vjp_finite_differences(torch_gpu):
     torch = torch_gpu
     X = _counts(300, 200, 0.06, seed=7)
