@@ -585,6 +585,40 @@ def _selected_matrix(torch, n, n_feat):
     return select_columns(X, sel)
 
 
+def operator_only(torch, chunks, cfg, reps, b=16):
+    """reps applications of the operator to a b-column block on the workload's
+    selected matrix (after three untimed ones), timed with HIP events on the
+    launching stream: the products alone, for counter runs."""
+    from snapatac2_amd.genome import synthetic_genome
+    from snapatac2_amd.pp import make_tile_matrix_chunked
+    from snapatac2_amd.pp._basic import _column_sums, _find_most_accessible_features
+    from snapatac2_amd.tl._operator import SpectralOperator, select_columns
+    X, _ = make_tile_matrix_chunked(chunks, synthetic_genome(), 500, EXCLUDE)
+    sel = np.sort(_find_most_accessible_features(_column_sums(X), 0.005, 0.005, cfg[1]))
+    Xs = select_columns(X, sel)
+    del X
+    op = SpectralOperator(Xs)
+    gen = torch.Generator(device="cuda").manual_seed(1)
+    V = torch.randn(Xs.shape[0], b, dtype=torch.float64, device="cuda", generator=gen)
+    Y = torch.empty_like(V)
+    for _ in range(3):
+        op.apply(V, Y)
+    st = torch.cuda.current_stream()  # the stream the operator launches on (_lib.stream_ptr)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        op.apply(V, Y)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    out = {"layout": op.layout, "columns": b, "ms_per_apply": round(ms, 3), "reps": reps, "nnz": int(Xs.nnz),
+           "alg_bytes": int(spmm_algorithmic_bytes(op, b))}
+    if op.layout == "panel":
+        out["packed"] = [op.panel_csr.packed, op.panel_csc.packed]
+        out["gathers_per_s"] = round(2 * Xs.nnz / (ms * 1e-3), 1)
+    return out
+
+
 def arpack_on_device(torch, n, cfg, parity: bool):
     """The reference's eigensolver call (scipy eigsh, which='LM', v0 =
     RandomState(0).rand(n), default ncv/tol; embedding.rs:155-171) on the FP64
@@ -748,6 +782,14 @@ def main():
                          "are also compared with this solver's")
     ap.add_argument("--ascent-steps", type=int, default=4,
                     help="eigengap-ascent steps of the untimed learned-weights quality check (0: skip it)")
+    ap.add_argument("--operator-only", type=int, default=0, metavar="REPS",
+                    help="time only REPS operator applications (16 columns) on the workload's selected "
+                         "matrix and print their average (its own JSON line); for profiling the products")
+    ap.add_argument("--operator-columns", type=int, default=16, metavar="B",
+                    help="block width of the --operator-only applications (1-32)")
+    ap.add_argument("--panel-pack-max", type=int, default=2, choices=[0, 1, 2],
+                    help="densest panel entry packing (2 selects the 16-column packed-8 product; "
+                         "1 or 0 the generic product)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one copy kernel of MIB MiB (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -760,6 +802,7 @@ def main():
     from snapatac2_amd.tl import _operator
     if args.layout:
         _operator.DEFAULT_TILED = args.layout
+    _operator.SpectralOperator.panel_pack_max = args.panel_pack_max
     torch.cuda.set_device(0)
     if args.copy_calibration:
         from snapatac2_amd import _lib
@@ -786,6 +829,10 @@ def main():
     torch.cuda.synchronize()
     n_frags = sum(f.n_fragments for _, f in chunks)
     log(f"[bench] {n_frags} fragments in {len(chunks)} chunks, {time.perf_counter() - t0:.1f}s")
+    if args.operator_only:
+        print(json.dumps({"operator": operator_only(torch, chunks, cfg, args.operator_only,
+                                                    args.operator_columns)}), flush=True)
+        return
     times = []
     info = emb = None
     spmm_events = []

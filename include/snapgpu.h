@@ -306,7 +306,13 @@ int snapgpu_spectral_vjp_w(const int64_t* d_indptr, const int32_t* d_idx, const 
  * [p * 32768, (p + 1) * 32768), so each product gathers from an L2-sized slice
  * of the dense operand; work is split into chunks of ~4096 entries aligned to
  * (row, panel) segments ("pairs"), whose FP32 sums are then added per row in
- * panel order. Counts below 65536 are packed with the column into 4 bytes.
+ * panel order. Integer counts are packed with the column into 4 bytes:
+ * `packed` = 2 for counts below 128 (the form the 16-column product gathers
+ * with the fewest instructions; needs dense_rows <= 2^26 for its 32-bit
+ * offsets, else the generic product runs), 1 for counts below 65536, 0 for
+ * float values or larger counts (a separate value array). snapgpu_panel_sizes
+ * returns the densest packing the values allow; snapgpu_panel_build accepts any
+ * packing the values fit and returns SNAPGPU_ERR_ARG for one they do not.
  * Build: snapgpu_panel_sizes, then allocate, then snapgpu_panel_build with the
  * same workspace (sizes leaves its tables there).
  *   ent / val       n_entries uint32 (val only when not packed)

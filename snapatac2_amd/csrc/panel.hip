@@ -16,10 +16,18 @@
 // panel order and applies the epilogue. No floating-point atomics, fixed
 // summation order: results are bitwise reproducible.
 //
-// Entry encoding (packed, counts < 65536): bits 0-14 column within the panel,
-// bit 15 first entry of a pair, bits 16-31 the count. Unpacked (float values or
-// large counts): column | first << 31 in `ent`, the raw 32-bit value in `val`.
+// Entry encodings (`packed`):
+//   2 (counts < 128): bits 6-20 column within the panel times 64 (the byte
+//     offset of its 16-float row), bits 24-30 the count, bit 31 first entry of
+//     a pair. The 16-column product adds the lane's column and the panel's
+//     base to get a 32-bit gather offset with one instruction, and converts the
+//     count with one more.
+//   1 (counts < 65536): bits 0-14 column within the panel, bit 15 first entry
+//     of a pair, bits 16-31 the count.
+//   0 (float values or larger counts): column | first << 31 in `ent`, the raw
+//     32-bit value in `val`.
 #include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "spectral_common.h"
@@ -42,6 +50,10 @@ constexpr int64_t kChunk = SNAPGPU_PANEL_CHUNK;  // entries per group-chunk (ali
 #define SNAPGPU_PANEL_INFLIGHT 8
 #endif
 constexpr int kInflight = SNAPGPU_PANEL_INFLIGHT;  // gathers in flight per lane
+#ifndef SNAPGPU_PANEL_INFLIGHT16
+#define SNAPGPU_PANEL_INFLIGHT16 8
+#endif
+constexpr int kInflight16 = SNAPGPU_PANEL_INFLIGHT16;  // the same, 16-column packed-8 product
 static_assert(kPanel <= 32768, "the packed entry holds a 15-bit column");
 
 // seg[row][p] = offset within the row of its first entry with idx >= p * kPanel
@@ -128,14 +140,15 @@ __global__ void pair_start_kernel(const int64_t* __restrict__ S, const int64_t* 
   if (q == Q || S[q + 1] > S[q]) pstart[S[q]] = E[q];
 }
 
-// One wave per row: every entry moves to its panel-major position.
-template <bool PACK>
+// One wave per row: every entry moves to its panel-major position. A count
+// that does not fit the packed encoding sets *bad.
+template <int MODE>
 __global__ void __launch_bounds__(256) scatter_kernel(const int64_t* __restrict__ ptr,
                                                       const int32_t* __restrict__ idx,
                                                       const uint32_t* __restrict__ val, int64_t nrows, int P,
                                                       const uint32_t* __restrict__ seg,
                                                       const int64_t* __restrict__ E, uint32_t* __restrict__ ent,
-                                                      uint32_t* __restrict__ val_out) {
+                                                      uint32_t* __restrict__ val_out, unsigned int* __restrict__ bad) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   for (int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; row < nrows; row += nwaves) {
@@ -146,8 +159,14 @@ __global__ void __launch_bounds__(256) scatter_kernel(const int64_t* __restrict_
       const int64_t i = (e - e0) - (int64_t)seg[row * (P + 1) + p];
       const int64_t dst = E[p * nrows + row] + i;
       const uint32_t local = (uint32_t)(col - p * kPanel);
-      if (PACK) {
-        ent[dst] = local | (i == 0 ? 0x8000u : 0u) | (val[e] << 16);
+      if (MODE == 2) {
+        const uint32_t v = val[e];
+        if (v >= 128u) atomicOr(bad, 1u);
+        ent[dst] = (local << 6) | ((v & 0x7Fu) << 24) | (i == 0 ? 0x80000000u : 0u);
+      } else if (MODE == 1) {
+        const uint32_t v = val[e];
+        if (v >= 65536u) atomicOr(bad, 1u);
+        ent[dst] = local | (i == 0 ? 0x8000u : 0u) | (v << 16);
       } else {
         ent[dst] = local | (i == 0 ? 0x80000000u : 0u);
         val_out[dst] = val[e];
@@ -190,13 +209,21 @@ __global__ void chunk_kernel(const int64_t* __restrict__ pstart, const int64_t* 
 // j % BW, register j / BW. Gathers are issued kInflight at a time. A pair's first entry
 // carries a flag: the running sum of the previous pair is stored and reset.
 // ---------------------------------------------------------------------------
-template <bool PACK, bool FV>
+template <int MODE, bool FV>
 __device__ __forceinline__ float entry_value(uint32_t e, uint32_t v) {
-  if (PACK) return (float)(e >> 16);
+  if (MODE == 2) return (float)((e >> 24) & 0x7Fu);
+  if (MODE == 1) return (float)(e >> 16);
   return FV ? __uint_as_float(v) : (float)v;
 }
 
-template <int BW, bool PACK, bool FV>
+template <int MODE>
+__device__ __forceinline__ uint32_t entry_column(uint32_t e) {
+  if (MODE == 2) return (e >> 6) & 0x7FFFu;
+  if (MODE == 1) return e & 0x7FFFu;
+  return e & 0x7FFFFFFFu;
+}
+
+template <int BW, int MODE, bool FV>
 __global__ void __launch_bounds__(256) product_kernel(const uint32_t* __restrict__ ent,
                                                       const uint32_t* __restrict__ val,
                                                       const int64_t* __restrict__ cstart,
@@ -206,8 +233,8 @@ __global__ void __launch_bounds__(256) product_kernel(const uint32_t* __restrict
                                                       float* __restrict__ partial) {
   constexpr int G = 64 / BW;
   constexpr int NB = 4 * BW;  // entries per group batch
-  constexpr uint32_t kFirst = PACK ? 0x8000u : 0x80000000u;
-  constexpr uint32_t kCol = PACK ? 0x7FFFu : 0x7FFFFFFFu;
+  constexpr bool PACK = MODE != 0;
+  constexpr uint32_t kFirst = MODE == 1 ? 0x8000u : 0x80000000u;
   const int lane = threadIdx.x & 63;
   const int g = lane / BW;
   const int c = lane % BW;
@@ -246,7 +273,7 @@ __global__ void __launch_bounds__(256) product_kernel(const uint32_t* __restrict
         const int j = j0 + u;
         ej[u] = (uint32_t)__shfl((int)e[j / BW], g * BW + j % BW, 64);
         vj[u] = PACK ? 0u : (uint32_t)__shfl((int)v[j / BW], g * BW + j % BW, 64);
-        z[u] = j < nvalid ? db[(int64_t)(ej[u] & kCol) * BW + c] : 0.f;
+        z[u] = j < nvalid ? db[(int64_t)entry_column<MODE>(ej[u]) * BW + c] : 0.f;
       }
 #pragma unroll
       for (int u = 0; u < kInflight; ++u) {
@@ -255,12 +282,143 @@ __global__ void __launch_bounds__(256) product_kernel(const uint32_t* __restrict
           have = true;
           acc = 0.f;
         }
-        acc = fmaf(entry_value<PACK, FV>(ej[u], vj[u]), z[u], acc);
+        acc = fmaf(entry_value<MODE, FV>(ej[u], vj[u]), z[u], acc);
       }
     }
     pos += NB;
   }
   if (have) partial[k * BW + c] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// The 16-column product over packed-8 entries, with the same work split and
+// summation order as product_kernel<16, 2, false> (so the same bits), and
+// fewer instructions per gather:
+//   - entry j of a group's 64-entry batch reaches the group's 16 lanes by a DPP
+//     row broadcast (the group is one 16-lane DPP row), not a cross-lane
+//     permute through LDS;
+//   - its gather offset is (entry & column field) | (panel base + 4 * lane
+//     column), a 32-bit offset from the dense operand's base (which therefore
+//     must be below 4 GiB: at most 2^26 rows of 16 floats);
+//   - its count is byte 3 of the entry (one conversion), the first-of-pair flag
+//     the sign bit (one compare);
+//   - batches that lie entirely inside the group-chunk skip the per-entry
+//     bounds test; the next batch's entries load while this one is gathered.
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ uint32_t row_bcast(uint32_t v) {
+  // lane N of each 16-lane row, to every lane of that row (DPP row_newbcast);
+  // every source lane is active, so no old value is needed
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x150 + N, 0xF, 0xF, true);
+}
+
+template <int J0, int... U>
+__device__ __forceinline__ void bcast_entries(uint32_t (&v)[sizeof...(U)], const uint32_t (&e)[4],
+                                              std::integer_sequence<int, U...>) {
+  ((v[U] = row_bcast<(J0 + U) % 16>(e[(J0 + U) / 16])), ...);
+}
+
+// INF entries of the batch, starting at J0. A first-of-pair entry carries
+// count + 128 in byte 3: its pair's sum starts at -128 * z, so that
+// fma(count + 128, z, -128 * z) rounds (count + 128) * z - 128 * z = count * z
+// exactly as the generic kernel's fma(count, z, 0) does (-128 * z is exact).
+// The per-entry flag tests run only when some lane of the wave meets a pair
+// boundary within these INF entries.
+template <int J0, int INF, bool TAIL>
+__device__ __forceinline__ void gather16_step(const uint32_t (&e)[4], const char* __restrict__ dense, uint32_t pbase,
+                                              int nvalid, float* __restrict__ partial, int c, int32_t& k,
+                                              bool& have, float& acc) {
+  uint32_t v[INF];
+  float z[INF];
+  bcast_entries<J0>(v, e, std::make_integer_sequence<int, INF>{});
+#pragma unroll
+  for (int u = 0; u < INF; ++u) {
+    const uint32_t off = (v[u] & 0x1FFFC0u) | pbase;
+    if (TAIL)
+      z[u] = J0 + u < nvalid ? *reinterpret_cast<const float*>(dense + off) : 0.f;
+    else
+      z[u] = *reinterpret_cast<const float*>(dense + off);
+  }
+  uint32_t any = 0;
+#pragma unroll
+  for (int u = 0; u < INF; ++u) any |= v[u];
+  if (__builtin_amdgcn_ballot_w64((int32_t)any < 0) == 0) {
+#pragma unroll
+    for (int u = 0; u < INF; ++u) acc = fmaf((float)(v[u] >> 24), z[u], acc);
+  } else {
+#pragma unroll
+    for (int u = 0; u < INF; ++u) {
+      if ((int32_t)v[u] < 0) {
+        if (have) partial[(int64_t)k++ * 16 + c] = acc;
+        have = true;
+        acc = -128.f * z[u];
+      }
+      acc = fmaf((float)(v[u] >> 24), z[u], acc);
+    }
+  }
+}
+
+template <int INF, bool TAIL, int... S>
+__device__ __forceinline__ void gather16_batch(const uint32_t (&e)[4], const char* __restrict__ dense,
+                                               uint32_t pbase, int nvalid, float* __restrict__ partial, int c,
+                                               int32_t& k, bool& have, float& acc, std::integer_sequence<int, S...>) {
+  (gather16_step<S * INF, INF, TAIL>(e, dense, pbase, nvalid, partial, c, k, have, acc), ...);
+}
+
+template <int INF>
+__global__ void __launch_bounds__(256) product16_kernel(const uint32_t* __restrict__ ent,
+                                                        const int64_t* __restrict__ cstart,
+                                                        const int32_t* __restrict__ cpair,
+                                                        const int32_t* __restrict__ cpanel, int64_t nchunks,
+                                                        const float* __restrict__ dense,
+                                                        float* __restrict__ partial) {
+  constexpr int NB = 64;  // entries per group batch
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4;
+  const int c = lane & 15;
+  const int64_t nb = gridDim.x, q8 = nb / 8, r8 = nb % 8;
+  const int64_t xcd = blockIdx.x % 8;
+  const int64_t blk = xcd * q8 + min(xcd, r8) + blockIdx.x / 8;
+  const int64_t chunk = ((blk * blockDim.x + threadIdx.x) >> 6) * 4 + g;
+  int64_t pos = 0, end = 0;
+  int32_t k = 0;  // pair index (pairs < 2^31)
+  uint32_t pbase = (uint32_t)c << 2;
+  if (chunk < nchunks) {
+    pos = cstart[chunk];
+    end = cstart[chunk + 1];
+    k = cpair[chunk];
+    pbase |= (uint32_t)cpanel[chunk] << 21;
+  }
+  const char* db = reinterpret_cast<const char*>(dense);
+  bool have = false;
+  float acc = 0.f;
+  uint32_t e[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int64_t q = pos + s * 16 + c;
+    e[s] = q < end ? __builtin_nontemporal_load(ent + q) : 0u;
+  }
+  while (__ballot(pos < end) != 0) {
+    if (pos < end) {
+      uint32_t en[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int64_t q = pos + NB + s * 16 + c;
+        en[s] = q < end ? __builtin_nontemporal_load(ent + q) : 0u;
+      }
+      if (pos + NB <= end) {
+        gather16_batch<INF, false>(e, db, pbase, NB, partial, c, k, have, acc,
+                                   std::make_integer_sequence<int, NB / INF>{});
+      } else {
+        gather16_batch<INF, true>(e, db, pbase, (int)(end - pos), partial, c, k, have, acc,
+                                  std::make_integer_sequence<int, NB / INF>{});
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) e[s] = en[s];
+      pos += NB;
+    }
+  }
+  if (have) partial[(int64_t)k * 16 + c] = acc;
 }
 
 // out row = sum of the row's pairs in panel order, then the epilogue:
@@ -308,15 +466,23 @@ static void launch_product(const snapgpu_panel_layout& L, const float* dense, fl
   if (waves > 0) {
     const unsigned grid = (unsigned)ceil_div(waves, 4);
     const bool fv = L.value_type == 1;
-    if (L.packed)
-      product_kernel<BW, true, false><<<grid, 256, 0, st>>>(L.ent, L.val, L.chunk_start, L.chunk_pair,
-                                                            L.chunk_panel, L.n_chunks, dense, partial);
+    // 32-bit gather offsets: the dense operand (BW floats per row) below 4 GiB
+    const bool off32 = L.dense_rows <= ((int64_t)1 << 26);
+    if (BW == 16 && L.packed == 2 && off32)
+      product16_kernel<kInflight16><<<grid, 256, 0, st>>>(L.ent, L.chunk_start, L.chunk_pair, L.chunk_panel,
+                                                          L.n_chunks, dense, partial);
+    else if (L.packed == 2)
+      product_kernel<BW, 2, false><<<grid, 256, 0, st>>>(L.ent, L.val, L.chunk_start, L.chunk_pair,
+                                                         L.chunk_panel, L.n_chunks, dense, partial);
+    else if (L.packed == 1)
+      product_kernel<BW, 1, false><<<grid, 256, 0, st>>>(L.ent, L.val, L.chunk_start, L.chunk_pair,
+                                                         L.chunk_panel, L.n_chunks, dense, partial);
     else if (fv)
-      product_kernel<BW, false, true><<<grid, 256, 0, st>>>(L.ent, L.val, L.chunk_start, L.chunk_pair,
-                                                            L.chunk_panel, L.n_chunks, dense, partial);
+      product_kernel<BW, 0, true><<<grid, 256, 0, st>>>(L.ent, L.val, L.chunk_start, L.chunk_pair,
+                                                        L.chunk_panel, L.n_chunks, dense, partial);
     else
-      product_kernel<BW, false, false><<<grid, 256, 0, st>>>(L.ent, L.val, L.chunk_start, L.chunk_pair,
-                                                             L.chunk_panel, L.n_chunks, dense, partial);
+      product_kernel<BW, 0, false><<<grid, 256, 0, st>>>(L.ent, L.val, L.chunk_start, L.chunk_pair,
+                                                         L.chunk_panel, L.n_chunks, dense, partial);
   }
   reduce_kernel<BW, EPI><<<(unsigned)ceil_div(L.n_rows * BW, 256), 256, 0, st>>>(
       L.row_pair_ptr, L.row_pair_ids, L.n_rows, partial, scale, out32, V, ldv, dinv, Y, ldy, b);
@@ -418,7 +584,7 @@ int snapgpu_panel_sizes(const int64_t* d_ptr, const int32_t* d_idx, const uint32
   for (int p = 0; p < P; ++p) nc += ceil_div(info[2 * (p + 1) + 1] - info[2 * p + 1], kChunk);
   *n_pairs = info[2 * P];
   *n_chunks = nc;
-  *packed = (value_type == 0 && maxc < 65536u) ? 1 : 0;
+  *packed = value_type != 0 ? 0 : maxc < 128u ? 2 : maxc < 65536u ? 1 : 0;
   return SNAPGPU_OK;
 }
 
@@ -430,6 +596,7 @@ int snapgpu_panel_build(const int64_t* d_ptr, const int32_t* d_idx, const uint32
   clear_error();
   const int P = snapgpu_panel_count(dense_rows);
   SG_ARG(ws_bytes >= build_ws_bytes(nrows, P, nnz), "workspace too small");
+  SG_ARG(packed >= 0 && packed <= 2, "packed must be 0, 1 or 2");
   SG_ARG(packed || d_val_out != nullptr, "unpacked layout needs a value array");
   hipStream_t st = as_stream(stream);
   BuildWs w = carve(d_ws, nrows, P, nnz);
@@ -461,14 +628,24 @@ int snapgpu_panel_build(const int64_t* d_ptr, const int32_t* d_idx, const uint32
                                                                       d_row_pair_ids);
   pair_start_kernel<<<(unsigned)ceil_div(Q + 1, 256), 256, 0, st>>>(w.S, w.E, Q, w.pstart);
   const unsigned gw = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nrows, 4), 16384));
-  if (packed)
-    scatter_kernel<true><<<gw, 256, 0, st>>>(d_ptr, d_idx, d_val, nrows, P, w.seg, w.E, d_ent, nullptr);
+  unsigned int* bad = w.maxc + 1;
+  SG_HIP(hipMemsetAsync(bad, 0, 4, st));
+  if (packed == 2)
+    scatter_kernel<2><<<gw, 256, 0, st>>>(d_ptr, d_idx, d_val, nrows, P, w.seg, w.E, d_ent, nullptr, bad);
+  else if (packed == 1)
+    scatter_kernel<1><<<gw, 256, 0, st>>>(d_ptr, d_idx, d_val, nrows, P, w.seg, w.E, d_ent, nullptr, bad);
   else
-    scatter_kernel<false><<<gw, 256, 0, st>>>(d_ptr, d_idx, d_val, nrows, P, w.seg, w.E, d_ent, d_val_out);
+    scatter_kernel<0><<<gw, 256, 0, st>>>(d_ptr, d_idx, d_val, nrows, P, w.seg, w.E, d_ent, d_val_out, bad);
   chunk_kernel<<<(unsigned)ceil_div(n_chunks + 1, 256), 256, 0, st>>>(w.pstart, w.info, P, n_chunks, K, nnz,
                                                                      d_chunk_start, d_chunk_pair, d_chunk_panel);
   SG_LAUNCH_CHECK();
+  unsigned int hbad = 0;
+  SG_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st));
   SG_HIP(hipStreamSynchronize(st));  // cinfo lives on this stack frame until the copy has run
+  if (hbad) {
+    set_error(packed == 2 ? "packed = 2 needs counts below 128" : "packed = 1 needs counts below 65536");
+    return SNAPGPU_ERR_ARG;
+  }
   return SNAPGPU_OK;
 }
 

@@ -74,6 +74,10 @@ def select_columns(X: DeviceCSR, cols: np.ndarray | None) -> DeviceCSR:
 class SpectralOperator:
     """Holds CSR/CSC counts, weights, scales and degrees of A on the device."""
 
+    # densest panel entry packing used (2: counts < 128, 1: counts < 65536,
+    # 0: separate values); lower it to run the generic panel product
+    panel_pack_max = 2
+
     def __init__(self, X: DeviceCSR, feature_weights=None, use_panels: bool | None = None,
                  layout: str | None = None):
         """layout: 'block' (LDS-tiled products, b <= 16 per launch), 'panel'
@@ -161,6 +165,7 @@ class SpectralOperator:
         _lib.check(L.snapgpu_panel_sizes(ptr.data_ptr(), idx.data_ptr(), val.data_ptr(), nrows, dense_rows,
                                          nnz, self.value_type, ws.data_ptr(), ws_b, C.byref(K),
                                          C.byref(NC), C.byref(packed), st))
+        packed = C.c_int32(min(packed.value, self.panel_pack_max))
         i32 = dict(dtype=torch.int32, device=dev)
         ent = torch.empty(max(nnz, 1), **i32)
         val_out = None if packed.value else torch.empty(max(nnz, 1), **i32)

@@ -160,10 +160,38 @@ def test_panel_segmented_operator(torch_gpu):
     assert torch.equal(op_p.matmat(Vt), op_p.matmat(Vt))
 
 
-@pytest.mark.parametrize("kind", ["small", "float", "large_counts", "empty_rows"])
+def test_panel_fast_product_bitwise(torch_gpu, monkeypatch):
+    """The 16-column product over counts < 128 (packed 2: DPP broadcast, 32-bit
+    gather offsets, first-of-pair sums started at -128 z) gives the same bits
+    as the generic product over the same entries (packed 1), with the same
+    work split and summation order; b = 8 and 32 take the generic product on
+    both encodings."""
+    torch = torch_gpu
+    from snapatac2_amd.tl._operator import SpectralOperator, _to_device_csr
+    X = small_counts(70000, 80000, 0.0015, seed=31)
+    X.data[::97] = 127  # the largest packed-2 count
+    Xd = _to_device_csr(X, torch)
+    op2 = SpectralOperator(Xd, use_panels=True)
+    monkeypatch.setattr(SpectralOperator, "panel_pack_max", 1)
+    op1 = SpectralOperator(Xd, use_panels=True)
+    assert op2.panel_csr.packed == 2 and op2.panel_csc.packed == 2
+    assert op1.panel_csr.packed == 1 and op1.panel_csc.packed == 1
+    rng = np.random.default_rng(4)
+    for b in (16, 11, 8, 32):
+        Vt = torch.from_numpy(rng.standard_normal((70000, b))).cuda()
+        assert torch.equal(op2.matmat(Vt), op1.matmat(Vt)), b
+    Xh, dinv, _, _ = osp.operator_parts(X)
+    V = rng.standard_normal((70000, 16))
+    ref = Xh @ (Xh.T @ V) - dinv[:, None] * V
+    Y = op2.matmat(torch.from_numpy(V).cuda()).cpu().numpy()
+    assert np.abs(Y - ref).max() / np.abs(ref).max() < 1e-5
+
+
+@pytest.mark.parametrize("kind", ["small", "float", "large_counts", "mid_counts", "empty_rows"])
 def test_panel_layout_variants(torch_gpu, kind):
-    """Panel layout on one panel (forced) and on several: packed counts, float
-    values and counts >= 65536 (unpacked), rows and columns without entries."""
+    """Panel layout on one panel (forced) and on several: counts < 128 (packed
+    2), counts in [128, 65536) (packed 1), float values and counts >= 65536
+    (unpacked), rows and columns without entries."""
     torch = torch_gpu
     from snapatac2_amd.tl._operator import SpectralOperator, _to_device_csr
     n, m = (3000, 5000) if kind == "small" else (40000, 36000)
@@ -176,6 +204,9 @@ def test_panel_layout_variants(torch_gpu, kind):
         # normalisation keeps the operator on the scale of the other cases
         X = X.astype(np.uint32)
         X.data[X.indptr[0]:X.indptr[50]] *= 70000
+    elif kind == "mid_counts":
+        X = X.astype(np.uint32)
+        X.data[X.indptr[10]:X.indptr[60]] *= 200
     elif kind == "empty_rows":
         X = X.tolil()
         X[100:140, :] = 0
@@ -185,7 +216,8 @@ def test_panel_layout_variants(torch_gpu, kind):
     Xd = _to_device_csr(X, torch)
     op = SpectralOperator(Xd, use_panels=True)
     ref_op = SpectralOperator(Xd, use_panels=False)
-    assert op.panel_csr.packed == (kind in ("small", "empty_rows"))
+    packed = {"small": 2, "empty_rows": 2, "mid_counts": 1, "float": 0, "large_counts": 0}[kind]
+    assert op.panel_csr.packed == packed and op.panel_csc.packed == packed
     Xh, dinv, _, _ = osp.operator_parts(X)
     rng = np.random.default_rng(3)
     for b in (16, 7, 32):
