@@ -346,14 +346,15 @@ def _barcodes(n):
     return ["".join(t) + "-1" for t in zip(*cols)]
 
 
-def fragment_import(torch, n_cells, cpu_cells, reps=3):
+def fragment_import(torch, n_cells, cpu_cells, reps=3, file_cells=2000):
     """Fragment-file import: n_cells synthetic cells exported as fragment text
     on the device (one line per fragment, rows in barcode order, as a file
     sorted by barcode), then imported with the text resident in HBM; once as
     is (sorted_by_barcode=True, the default) and once with the lines in a
     random order (sorted_by_barcode=False). Both must give the fragments back.
     CPU baseline: the one-thread C++ restatement on the text of the first
-    cpu_cells cells."""
+    cpu_cells cells. With file_cells, also import_fragments from files
+    (fragment_import_file)."""
     from snapatac2_amd import synth
     from snapatac2_amd._device import DeviceFragments
     from snapatac2_amd.genome import synthetic_genome
@@ -400,6 +401,54 @@ def fragment_import(torch, n_cells, cpu_cells, reps=3):
                                          f"sorted by barcode", "s": round(dt, 2), "agrees": bool(ok)}
         out["vs_cpu"] = round(out["sorted"]["GB_per_s"] / out["cpu_baseline"]["value"], 1)
         log(f"[bench] import CPU baseline: {dt:.2f} s for {nb / 1e9:.2f} GB")
+    if file_cells:
+        out["file"] = fragment_import_file(torch, fr, cs, bcs, min(file_cells, n_cells))
+    return out
+
+
+def fragment_import_file(torch, fr, cs, bcs, m):
+    """import_fragments(path) from a file to the device rows: the text of the
+    first m cells written to a plain and to a gzip (level 1) file in a temporary
+    directory, each imported once untimed (page cache warm) and then timed. The
+    host reads (and gunzips) the file; the device parses the blocks."""
+    import gzip
+    import shutil
+    import tempfile
+    from snapatac2_amd._device import DeviceFragments
+    from snapatac2_amd.pp import export_fragment_text, import_fragments
+    nf = int(fr.cell_ptr[m].item())
+    sub = DeviceFragments(fr.cell_ptr[:m + 1], fr.pos[:nf], fr.values[:nf], fr.n_cols, True)
+    text, nb = export_fragment_text(sub, cs, bcs[:m])
+    host = text[:nb].cpu().numpy().tobytes()
+    del text
+    d = tempfile.mkdtemp(prefix="snapgpu_import_")
+    out = {"workload": f"first {m} cells, {nf} lines, {nb / 1e9:.3f} GB of text"}
+    try:
+        paths = {"plain": os.path.join(d, "fragments.tsv"), "gzip": os.path.join(d, "fragments.tsv.gz")}
+        with open(paths["plain"], "wb") as f:
+            f.write(host)
+        with gzip.open(paths["gzip"], "wb", compresslevel=1) as f:
+            f.write(host)
+        del host
+        for label, path in paths.items():
+            ts, ok = [], False
+            for _ in range(2):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                ad = import_fragments(path, cs, min_num_fragments=0)
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+                g = ad.obsm["fragment_paired"]
+                ok = (list(ad.obs_names) == bcs[:m] and torch.equal(g.cell_ptr, sub.cell_ptr)
+                      and torch.equal(g.pos, sub.pos) and torch.equal(g.values, sub.values))
+                del ad, g
+            dt = ts[-1]
+            out[label] = {"s": round(dt, 3), "file_bytes": os.path.getsize(path),
+                          "text_GB_per_s": round(nb / dt / 1e9, 2), "fragments_per_s": round(nf / dt, 1),
+                          "round_trip": bool(ok)}
+            log(f"[bench] import from a {label} file: {dt:.2f} s for {nb / 1e9:.2f} GB of text, round trip {ok}")
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
     return out
 
 

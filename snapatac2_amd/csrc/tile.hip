@@ -178,13 +178,33 @@ __device__ __forceinline__ int lds_rank(const uint32_t* arr, int n, uint32_t key
   return lo;
 }
 
-// Fast path for the per-fragment event pairs (k[0] = first event of lane i's
+// Number of entries below key (STRICT) or at or below key in a sorted LDS
+// array of n <= 64 entries: a fixed seven-step search, no divergent loop.
+template <bool STRICT>
+__device__ __forceinline__ int lds_count64(const uint32_t* arr, int n, uint32_t key) {
+  int lo = 0;
+#pragma unroll
+  for (int step = 64; step > 0; step >>= 1) {
+    if (lo + step <= n) {
+      const uint32_t v = arr[lo + step - 1];
+      if (STRICT ? v < key : v <= key) lo += step;
+    }
+  }
+  return lo;
+}
+
+// Fast paths for the per-fragment event pairs (k[0] = first event of lane i's
 // fragment, k[1] = its second event or kInf; kInf in k[0] = no event). Start
 // bins never decrease along a row and the second event is >= the first, so the
 // pair order (lane 0 first, lane 0 second, lane 1 first, ...) is sorted unless
 // some fragment ends in a later bin than where the next fragment starts. When
-// it is sorted, the events are placed at their ranks (elements 0..63 in k[0],
-// 64..127 in k[1], kInf last) and true is returned; otherwise k is unchanged.
+// it is sorted, the events are placed at their ranks. Otherwise (several
+// fragments at one locus, most tiles of real and generated rows) the first
+// events still ascend along the lanes and the second events usually do too:
+// then the two ascending lists are merged by rank (a first event before a
+// second event of the same bin). Either way the events end sorted (elements
+// 0..63 in k[0], 64..127 in k[1], kInf last) and true is returned; when neither
+// list order holds, k is unchanged and false is returned (bitonic sort).
 template <int CAP>
 __device__ __forceinline__ bool presort_pairs(uint32_t (&k)[2], int lane, WaveLds<CAP>* L) {
   const bool v0 = k[0] != kInf, v1 = k[1] != kInf;
@@ -193,7 +213,29 @@ __device__ __forceinline__ bool presort_pairs(uint32_t (&k)[2], int lane, WaveLd
   const int nl = later ? lowbit64(later) : lane;
   const uint32_t nxt = __shfl(k[0], nl, 64);
   const uint32_t last = v1 ? k[1] : k[0];
-  if (ballot64(v0 && later != 0 && last > nxt) != 0) return false;
+  if (ballot64(v0 && later != 0 && last > nxt) != 0) {
+    const uint64_t later1 = m1 & lanemask_gt(lane);
+    const uint32_t nxt1 = __shfl(k[1], later1 ? lowbit64(later1) : lane, 64);
+    if (ballot64((v0 && later != 0 && k[0] > nxt) || (v1 && later1 != 0 && k[1] > nxt1)) != 0) return false;
+    const uint64_t lt = lanemask_lt(lane);
+    const int n0 = popc64(m0), n1 = popc64(m1);
+    const int i0 = popc64(m0 & lt), i1 = popc64(m1 & lt);
+    if (v0) L->nbin[i0] = k[0];
+    if (v1) L->nbin[64 + i1] = k[1];
+    wave_sync();
+    const int r0 = v0 ? i0 + lds_count64<true>(L->nbin + 64, n1, k[0]) : 0;
+    const int r1 = v1 ? i1 + lds_count64<false>(L->nbin, n0, k[1]) : 0;
+    L->mbin[lane] = kInf;
+    L->mbin[64 + lane] = kInf;
+    wave_sync();
+    if (v0) L->mbin[r0] = k[0];
+    if (v1) L->mbin[r1] = k[1];
+    wave_sync();
+    k[0] = L->mbin[lane];
+    k[1] = L->mbin[64 + lane];
+    wave_sync();
+    return true;
+  }
   const uint64_t lt = lanemask_lt(lane);
   const int r0 = popc64(m0 & lt) + popc64(m1 & lt);
   L->nbin[lane] = kInf;

@@ -109,6 +109,58 @@ def test_dense_duplicates_and_long_spans(torch_gpu):
             _same(m, o)
 
 
+def _ordered_cells(rng, cs):
+    """Rows that exercise the in-order tile path and its hand-over to the
+    general merge: runs that span whole 64-fragment tiles, fragments whose last
+    bin is the next fragment's first, occasional interleaved pairs (a fragment
+    ending beyond the next one's start bin), size-filtered and excluded-chromosome
+    fragments inside runs, and tiles that cross chromosome boundaries."""
+    starts = np.cumsum([0] + list(cs.values()))
+    names = list(cs)
+    cells = []
+    # one bin for 300 fragments (the pending run carried through whole tiles)
+    cells.append([(1000, 40)] * 300)
+    # steady walk: each fragment ends in the bin where the next one starts
+    cells.append([(100 * i, 101) for i in range(400)])
+    # mostly ordered with interleaved pairs every ~70 fragments
+    c, x = [], 5000
+    for i in range(700):
+        x += int(rng.integers(0, 60))
+        ln = 900 if i % 70 == 69 else int(rng.integers(20, 120))
+        c.append((x, ln))
+    cells.append(sorted(c))
+    # walk over every chromosome boundary, with short (size-filtered) fragments
+    c = []
+    for ci, nm in enumerate(names):
+        lo, hi = starts[ci], starts[ci + 1]
+        for p in range(int(lo), int(hi) - 400, max(1, (int(hi) - int(lo)) // 150)):
+            c.append((p, 5 if (p // 7) % 9 == 0 else int(rng.integers(30, 300))))
+    cells.append(sorted(c))
+    # dense peaks: many fragments per bin, 1-3 bins long
+    c = []
+    for _ in range(40):
+        ci = int(rng.integers(0, len(names)))
+        centre = int(rng.integers(starts[ci] + 200, starts[ci + 1] - 1000))
+        for _ in range(int(rng.integers(20, 90))):
+            c.append((centre + int(rng.integers(-150, 150)), int(rng.integers(40, 700))))
+    cells.append(sorted(c))
+    # short fragments (both insertions in one bin) in long runs
+    cells.append(sorted([(2000 + 3 * (i // 20), 2) for i in range(500)]))
+    return cells
+
+
+@pytest.mark.parametrize("strategy", ["insertion", "paired-insertion"])
+def test_ordered_tiles_and_handover(torch_gpu, strategy):
+    rng = np.random.default_rng(23)
+    cs = {"chr1": 60000, "chr2": 40000, "chrM": 5000, "chr3": 30000}
+    cells = _ordered_cells(rng, cs)
+    for bin_size, excl, lo, hi in [(500, ["chrM"], None, None), (1, [], None, None), (7, ["chr2"], 10, 650),
+                                   (100, [], 30, None), (20000, ["chrM"], None, None)]:
+        m, names, (cp, pos, ln) = _tile(torch_gpu, cells, cs, bin_size, excl, lo, hi, strategy)
+        o = otile.tile_matrix(cp, pos, ln, cs, bin_size, excl, lo, hi, strategy)
+        _same(m, o)
+
+
 def test_synthetic_2k_cells_all_strategies(torch_gpu):
     from snapatac2_amd import synth
     from snapatac2_amd.genome import synthetic_genome
