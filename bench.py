@@ -324,8 +324,7 @@ def type_separation(torch, E, types):
     1 when the types are well apart; this ratio keeps growing with the margin."""
     t = types.to(E.device).long()
     nt = int(t.max().item()) + 1
-    cnt = torch.bincount(t, minlength=nt).to(E.dtype)
-    mu = torch.zeros((nt, E.shape[1]), dtype=E.dtype, device=E.device).index_add_(0, t, E) / cnt[:, None]
+    mu = torch.stack([E[t == i].mean(0) for i in range(nt)])  # not index_add_: atomics on ten rows
     d = torch.cdist(E, mu)
     own = d.gather(1, t[:, None]).squeeze(1)
     d.scatter_(1, t[:, None], float("inf"))

@@ -20,7 +20,7 @@ from typing import NamedTuple
 import numpy as np
 
 from .. import _lib
-from ._lanczos import eigsh_lm
+from ._lanczos import _Linalg, eigsh_lm
 from ._operator import SpectralOperator
 
 
@@ -139,7 +139,8 @@ def _pair_f64(op: SpectralOperator, U, evals, k: int):
     AV = op.apply_f64(V)
     if AV is None:
         return V, (float(evals[k - 1]), float(evals[k]))
-    H = (V.T @ AV).cpu().numpy()
+    la = _Linalg(op.torch, V.shape[0], 2, V.device)
+    H = la.tn(V, AV).cpu().numpy()  # the FP64 tall-skinny product (V^T AV, 2 x 2)
     H = 0.5 * (H + H.T)
     lam, Q = np.linalg.eigh(H)
     order = np.argsort(-np.abs(lam), kind="stable")
