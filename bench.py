@@ -137,9 +137,16 @@ def spmm_algorithmic_bytes(op, b):
     packed layouts, 8 for separate index and count arrays) and, as the pointer
     arrays, whatever the layout reads instead of indptr (tile starts and row
     tables for the block layout, chunk tables and row -> pair lists for the
-    panel layout, int64 row pointers otherwise).
+    panel layout, bucket starts, quad chunk tables and row slots for the window
+    layout, int64 row pointers otherwise).
     V read, Z written and read, Y written in b-wide rows; dinv and r per cell."""
     n, m, nnz = op.n, op.m, op.X.nnz
+    if getattr(op, "layout", "panel") == "window":
+        ptrs = 0
+        for L in (op.window_csr, op.window_csc):
+            nbk = L.n_blocks * L.n_windows
+            ptrs += (nbk + 1) * 8 + nbk * 257 * 4 + (L.n_blocks + 1) * 8 + L.n_rows * 4
+        return 2 * nnz * 4 + ptrs + 4 * b * (2 * n + 2 * m) + 8 * n
     if getattr(op, "layout", "panel") == "block":
         ptrs = 0
         e = 0
@@ -823,7 +830,7 @@ def main():
                     help="skip the cell x peak and cell x gene matrix measurements")
     ap.add_argument("--import-only", action="store_true",
                     help="run only the fragment-import measurement (its own JSON line)")
-    ap.add_argument("--layout", default=None, choices=["block", "panel"],
+    ap.add_argument("--layout", default=None, choices=["window", "block", "panel"],
                     help="operator layout of the large sparse products (default: the package default)")
     ap.add_argument("--parity-cells", type=int, default=100_000,
                     help="cells of the reference-eigsh run in the CPU baseline leg, whose eigenpairs "

@@ -356,7 +356,58 @@ int snapgpu_spectral_apply_panel(const snapgpu_panel_layout* csr, const snapgpu_
                                  const double* d_V, int64_t ldv, int32_t b, double* d_Y,
                                  int64_t ldy, void* d_ws, int64_t ws_bytes, void* stream);
 
-/* Block-tiled layout (the faster form of the same products for b <= 16). One
+/* Window layout (the same products as snapgpu_spectral_apply, b <= 16, counts
+ * below 256; replaces the matvec of the reference's eigensolver call,
+ * snapatac2-core/src/embedding.rs:155-171 via the operator of :269-326). One
+ * workgroup keeps the sums of a block of output rows in 2048 LDS slots and walks
+ * the dense operand in windows of snapgpu_window_count's 1024 rows, so the
+ * gathers of one window reuse L1; within a window a row's entries are cut into pieces of at most 16, each
+ * with its own slot, and a row's slots are added in order at the end: fixed
+ * summation order. Build: snapgpu_window_plan (slots per row, workgroups;
+ * returns the block rows and row slots on the host), upload them, then
+ * snapgpu_window_build.
+ *   ent            n_entries uint32: column in window | slot << 10 | piece start
+ *                  << 21 | count << 24
+ *   bucket_start   n_blocks * n_windows + 1 int64 (bucket = (block, window))
+ *   quad_off       n_blocks * n_windows * 257 uint32 (piece-aligned chunk of
+ *                  each of the 256 quads of a workgroup, relative to its bucket)
+ *   block_row      n_blocks + 1 int64 (first row of each workgroup)
+ *   row_slot       n_rows uint32 (first slot | number of slots << 16) */
+typedef struct snapgpu_window_layout {
+  const uint32_t* ent;
+  const int64_t* bucket_start;
+  const uint32_t* quad_off;
+  const int64_t* block_row;
+  const uint32_t* row_slot;
+  int64_t n_blocks;
+  int64_t n_windows;
+  int64_t n_rows;
+  int64_t dense_rows;
+  int64_t n_entries;
+} snapgpu_window_layout;
+
+int64_t snapgpu_window_count(int64_t dense_rows); /* windows of 1024 dense rows */
+int64_t snapgpu_window_plan_workspace_bytes(int64_t nrows);
+int snapgpu_window_plan(const int64_t* d_ptr, const int32_t* d_idx, const uint32_t* d_val,
+                        int64_t nrows, int64_t dense_rows, int64_t nnz, void* d_ws,
+                        int64_t ws_bytes, int64_t* h_block_row, int64_t max_blocks,
+                        int64_t* n_blocks, uint32_t* h_row_slot, void* stream);
+int64_t snapgpu_window_build_workspace_bytes(int64_t n_blocks, int64_t dense_rows);
+int snapgpu_window_build(const int64_t* d_ptr, const int32_t* d_idx, const uint32_t* d_val,
+                         int64_t nrows, int64_t dense_rows, int64_t nnz,
+                         const int64_t* d_block_row, int64_t n_blocks, const uint32_t* d_row_slot,
+                         void* d_ws, int64_t ws_bytes, uint32_t* d_ent, int64_t* d_bucket_start,
+                         uint32_t* d_quad_off, void* stream);
+/* Y = A V over the two window layouts (csr: rows = cells, dense_rows = features;
+ * csc: rows = features, dense_rows = cells), b <= 16. */
+int64_t snapgpu_spectral_apply_window_workspace_bytes(int64_t n, int64_t m);
+int snapgpu_spectral_apply_window(const snapgpu_window_layout* csr, const snapgpu_window_layout* csc,
+                                  const float* d_r32, const float* d_w2, const double* d_dinv,
+                                  const double* d_V, int64_t ldv, int32_t b, double* d_Y,
+                                  int64_t ldy, void* d_ws, int64_t ws_bytes, void* stream);
+
+/* Block-tiled layout (an LDS-staged form of the same products for b <= 16,
+ * slower than the panel layout at a million cells). One
  * workgroup owns a block of output rows (rows ordered by decreasing entry
  * count, blocks of about equal work) and keeps their sums in LDS; the dense
  * operand is swept in panels of snapgpu_block_panel_rows() rows, each read

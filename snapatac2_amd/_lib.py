@@ -73,6 +73,13 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_block_build_workspace_bytes": (_i64, [_i64]),
     "snapgpu_block_build": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _i32, _p, _i64, _i64, _i64, _p, _i64, _p, _p,
                                       _p, _p, _p, _p, _p, _p, _p]),
+    "snapgpu_window_count": (_i64, [_i64]),
+    "snapgpu_window_plan_workspace_bytes": (_i64, [_i64]),
+    "snapgpu_window_plan": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _p, _i64, _p, _i64, C.POINTER(_i64), _p, _p]),
+    "snapgpu_window_build_workspace_bytes": (_i64, [_i64, _i64]),
+    "snapgpu_window_build": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _p, _i64, _p, _p, _i64, _p, _p, _p, _p]),
+    "snapgpu_spectral_apply_window_workspace_bytes": (_i64, [_i64, _i64]),
+    "snapgpu_spectral_apply_window": (C.c_int, [_p, _p, _p, _p, _p, _p, _i64, _i32, _p, _i64, _p, _i64, _p]),
     "snapgpu_spectral_apply_block_workspace_bytes": (_i64, [_i64, _i64]),
     "snapgpu_spectral_apply_block": (C.c_int, [_p, _p, _p, _p, _p, _p, _i64, _i32, _p, _i64, _p, _i64, _p]),
     "snapgpu_gemm_tn_workspace_bytes": (_i64, [_i64, _i32, _i32]),
@@ -129,6 +136,13 @@ class BlockLayout(C.Structure):
                 ("n_entries", _i64), ("packed", _i32), ("value_type", _i32)]
 
 
+class WindowLayout(C.Structure):
+    """snapgpu_window_layout (include/snapgpu.h): device pointers and sizes."""
+    _fields_ = [("ent", _p), ("bucket_start", _p), ("quad_off", _p), ("block_row", _p), ("row_slot", _p),
+                ("n_blocks", _i64), ("n_windows", _i64), ("n_rows", _i64), ("dense_rows", _i64),
+                ("n_entries", _i64)]
+
+
 class PeakIndex(C.Structure):
     """snapgpu_peak_index (include/snapgpu.h): device pointers, the chromosome and column counts."""
     _fields_ = [("start", _p), ("end", _p), ("pmax", _p), ("orig", _p), ("chrom_off", _p), ("base", _p),
@@ -153,6 +167,10 @@ class SnapGpuError(RuntimeError):
 
 class PanicException(SnapGpuError):
     """Raised where the reference Rust code panics on invalid data."""
+
+
+class CapacityError(SnapGpuError):
+    """A fixed capacity of a kernel or layout was exceeded (SNAPGPU_ERR_CAPACITY)."""
 
 
 _lib = None
@@ -182,6 +200,8 @@ def check(rc: int) -> None:
         raise ValueError(msg)
     if rc == ERR_DATA:
         raise PanicException(msg)
+    if rc == ERR_CAPACITY:
+        raise CapacityError(msg)
     raise SnapGpuError(msg)
 
 

@@ -130,11 +130,24 @@ class SpectralOperator:
         if layout is None:
             tiled = (pc > 1 or pr > 1) if use_panels is None else bool(use_panels)
             layout = DEFAULT_TILED if tiled else "csr"
-        if layout not in ("block", "panel", "csr"):
-            raise ValueError("layout must be 'block', 'panel' or 'csr'")
+        if layout not in ("window", "block", "panel", "csr"):
+            raise ValueError("layout must be 'window', 'block', 'panel' or 'csr'")
+        if layout == "window" and self.value_type != 0:
+            layout = "panel"  # the window entries hold integer counts
         self.layout = layout
         self.use_panels = layout != "csr"
         self._panel_keep = []
+        if layout == "window":
+            try:
+                self.window_csr = self._build_window(X.indptr, X.indices, X.data, n, m, X.nnz)
+                self.window_csc = self._build_window(self.colptr, self.rowidx, self.cval, m, n, X.nnz)
+            except _lib.CapacityError:
+                # counts >= 256 or too many windows: the panel layout takes them
+                self._panel_keep = []
+                layout = self.layout = "panel"
+            if layout == "window":
+                self._pws_b = int(L.snapgpu_spectral_apply_window_workspace_bytes(n, m))
+                self._pws = torch.empty(self._pws_b, dtype=torch.uint8, device=dev)
         if layout == "panel":
             self.panel_csr = self._build_panel(X.indptr, X.indices, X.data, n, m, X.nnz)
             self.panel_csc = self._build_panel(self.colptr, self.rowidx, self.cval, m, n, X.nnz)
@@ -184,6 +197,38 @@ class SpectralOperator:
                                 cstart.data_ptr(), cpair.data_ptr(), cpanel.data_ptr(), NC.value,
                                 rp_ptr.data_ptr(), rp_ids.data_ptr(), K.value, nrows, dense_rows,
                                 packed.value, self.value_type)
+
+    def _build_window(self, ptr, idx, val, nrows: int, dense_rows: int, nnz: int):
+        """Window layout of one storage direction (see snapgpu_window_plan/build)."""
+        torch = self.torch
+        L = _lib.lib()
+        dev = ptr.device
+        st = _lib.stream_ptr()
+        ws_b = int(L.snapgpu_window_plan_workspace_bytes(nrows))
+        ws = torch.empty(ws_b, dtype=torch.uint8, device=dev)
+        max_blocks = nrows + 1
+        h_block_row = np.zeros(max_blocks + 1, dtype=np.int64)
+        h_row_slot = np.zeros(nrows, dtype=np.uint32)
+        nb = C.c_int64()
+        _lib.check(L.snapgpu_window_plan(ptr.data_ptr(), idx.data_ptr(), val.data_ptr(), nrows, dense_rows, nnz,
+                                         ws.data_ptr(), ws_b, h_block_row.ctypes.data, max_blocks, C.byref(nb),
+                                         h_row_slot.ctypes.data, st))
+        del ws
+        nb = nb.value
+        block_row = torch.from_numpy(h_block_row[:nb + 1]).to(dev)
+        row_slot = torch.from_numpy(h_row_slot.view(np.int32)).to(dev)
+        nw = int(L.snapgpu_window_count(dense_rows))
+        bws_b = int(L.snapgpu_window_build_workspace_bytes(nb, dense_rows))
+        bws = torch.empty(bws_b, dtype=torch.uint8, device=dev)
+        ent = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+        bstart = torch.empty(nb * nw + 1, dtype=torch.int64, device=dev)
+        qoff = torch.empty(nb * nw * 257, dtype=torch.int32, device=dev)
+        _lib.check(L.snapgpu_window_build(ptr.data_ptr(), idx.data_ptr(), val.data_ptr(), nrows, dense_rows, nnz,
+                                          block_row.data_ptr(), nb, row_slot.data_ptr(), bws.data_ptr(), bws_b,
+                                          ent.data_ptr(), bstart.data_ptr(), qoff.data_ptr(), st))
+        self._panel_keep.append((ent, bstart, qoff, block_row, row_slot))
+        return _lib.WindowLayout(ent.data_ptr(), bstart.data_ptr(), qoff.data_ptr(), block_row.data_ptr(),
+                                 row_slot.data_ptr(), nb, nw, nrows, dense_rows, nnz)
 
     def _build_block(self, ptr, idx, val, nrows: int, dense_rows: int, nnz: int,
                      target_blocks: int = BLOCK_TARGET):
@@ -281,7 +326,7 @@ class SpectralOperator:
         L = _lib.lib()
         X = self.X
         b = V.shape[1] if b is None else b
-        wmax = 16 if self.layout == "block" else 32
+        wmax = 16 if self.layout in ("block", "window") else 32
         if b > wmax:
             for c0 in range(0, b, wmax):
                 self.apply(V[:, c0:c0 + wmax], Y[:, c0:c0 + wmax])
@@ -290,7 +335,12 @@ class SpectralOperator:
         if self.events is not None:
             ev = (self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True))
             ev[0].record()
-        if self.layout == "block":
+        if self.layout == "window":
+            _lib.check(L.snapgpu_spectral_apply_window(
+                C.byref(self.window_csr), C.byref(self.window_csc), self.r32.data_ptr(),
+                self.w2.data_ptr(), self.dinv.data_ptr(), V.data_ptr(), V.stride(0), b, Y.data_ptr(),
+                Y.stride(0), self._pws.data_ptr(), self._pws_b, _lib.stream_ptr()))
+        elif self.layout == "block":
             _lib.check(L.snapgpu_spectral_apply_block(
                 C.byref(self.block_csr), C.byref(self.block_csc), self.r32.data_ptr(),
                 self.w2.data_ptr(), self.dinv.data_ptr(), V.data_ptr(), V.stride(0), b, Y.data_ptr(),

@@ -17,6 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 STRUCTS = {
     "snapgpu_panel_layout": _lib.PanelLayout,
     "snapgpu_block_layout": _lib.BlockLayout,
+    "snapgpu_window_layout": _lib.WindowLayout,
     "snapgpu_peak_index": _lib.PeakIndex,
     "snapgpu_frag_options": _lib.FragOptions,
     "snapgpu_frag_summary": _lib.FragSummary,
