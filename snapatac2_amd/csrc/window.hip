@@ -397,9 +397,11 @@ int snapgpu_window_plan(const int64_t* d_ptr, const int32_t* d_idx, const uint32
   clear_error();
   SG_ARG(nrows >= 1 && dense_rows >= 1 && nnz >= 0, "bad shape");
   SG_ARG(ws_bytes >= snapgpu_window_plan_workspace_bytes(nrows), "workspace too small");
-  SG_ARG(windows_of(dense_rows) <= kMaxWindows, "dense operand has too many windows for the window layout");
-  SG_ARG(dense_rows <= (int64_t(1) << 26), "dense operand too large for 32-bit window offsets");
   SG_ARG(h_block_row != nullptr && n_blocks != nullptr && h_row_slot != nullptr, "null output");
+  if (windows_of(dense_rows) > kMaxWindows || dense_rows > (int64_t(1) << 26)) {
+    set_error("window layout: the dense operand has more rows than the layout's windows and 32-bit offsets hold");
+    return SNAPGPU_ERR_CAPACITY;
+  }
   hipStream_t st = as_stream(stream);
   char* q = (char*)(((uintptr_t)d_ws + 255) & ~(uintptr_t)255);
   int32_t* s = (int32_t*)q;
@@ -460,7 +462,10 @@ int snapgpu_window_build(const int64_t* d_ptr, const int32_t* d_idx, const uint3
   clear_error();
   SG_ARG(nrows >= 1 && n_blocks >= 1 && nnz >= 0, "bad shape");
   SG_ARG(ws_bytes >= snapgpu_window_build_workspace_bytes(n_blocks, dense_rows), "workspace too small");
-  SG_ARG(windows_of(dense_rows) <= kMaxWindows, "dense operand has too many windows for the window layout");
+  if (windows_of(dense_rows) > kMaxWindows) {
+    set_error("window layout: the dense operand has more rows than the layout's windows hold");
+    return SNAPGPU_ERR_CAPACITY;
+  }
   hipStream_t st = as_stream(stream);
   const int64_t nw = windows_of(dense_rows);
   const int64_t nbk = n_blocks * nw;

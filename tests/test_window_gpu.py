@@ -138,6 +138,27 @@ def test_window_fallbacks(torch_gpu):
     assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-5
 
 
+def test_window_empty_rows_columns_and_one_window(torch_gpu):
+    """Rows and columns without entries (zero slots; zero sums), and a matrix
+    whose dense operands fit one window on both sides."""
+    torch = torch_gpu
+    for n, m in ((900, 700), (5000, 3000)):
+        X = _counts(n, m, 0.01, seed=n).tolil()
+        X[10:40, :] = 0
+        X[:, 100:130] = 0
+        X = sp.csr_matrix(X, dtype=np.uint32)
+        X.eliminate_zeros()
+        op, op_p = _ops(torch, X)
+        assert op.layout == "window"
+        Xh, dinv, _, _ = osp.operator_parts(X)
+        V = np.random.default_rng(1).standard_normal((n, 16))
+        ref = Xh @ (Xh.T @ V) - dinv[:, None] * V
+        got = op.matmat(torch.from_numpy(V).cuda()).cpu().numpy()
+        assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-5
+        rs = op._panel_keep[0][4].cpu().numpy().view(np.uint32)
+        assert (rs[10:40] >> 16 == 0).all()  # empty rows own no slot
+
+
 def test_window_eigenpairs(torch_gpu):
     """The block Lanczos solver on the window layout gives the panel layout's
     eigenvalues (1e-7 relative; both within the operator's FP32 rounding)."""
