@@ -11,6 +11,7 @@
 #include <algorithm>
 
 #include "common.h"
+#include "spectral_common.h"
 
 namespace snapgpu {
 namespace nys {
@@ -65,10 +66,12 @@ __global__ void __launch_bounds__(kThreads) row_inv_norm_kernel(const int64_t* _
   const int64_t row = ((int64_t)blockIdx.x * kThreads + threadIdx.x) >> 6;
   if (row >= n) return;
   double acc = 0.0;
-  for (int64_t e = indptr[row] + lane; e < indptr[row + 1]; e += 64) {
-    const double x = (double)cnt[e] * w[idx[e]];
-    acc = fma(x, x, acc);
-  }
+  spectral::lane_entries<false>(
+      idx, cnt, indptr[row] + lane, indptr[row + 1], acc, [&](int32_t j) { return w[j]; },
+      [](double& a, double v, double wj) {
+        const double x = v * wj;
+        a = fma(x, x, a);
+      });
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if (lane == 0) s[row] = acc > 0.0 ? 1.0 / sqrt(acc) : 0.0;
 }
@@ -78,13 +81,48 @@ __global__ void __launch_bounds__(kThreads) row_inv_norm_kernel(const int64_t* _
 // 64 / KP lane groups take every (64 / KP)-th entry; the groups' sums are added
 // by a butterfly over the group bits. KP is the smallest power of two >= k (at
 // most 32), so a narrow block keeps every lane busy; the summation order is
-// fixed for a given k.
-template <int KP>
+// fixed for a given k. A lane's entries are taken kBatch at a time: their
+// indices and counts are loaded first, then every gather, then the multiply-adds
+// in entry order, so kBatch gathers per lane are in flight instead of one after
+// each dependent index load. Lanes past column k gather column k - 1 (in
+// bounds, never stored), which keeps every load unconditional.
+constexpr int kBatch = 8;
+
+template <int KP, bool CS>
+__device__ __forceinline__ double scaled_row_sum(const int32_t* __restrict__ idx, const uint32_t* __restrict__ cnt,
+                                                 const double* __restrict__ b, const double* __restrict__ B,
+                                                 int64_t ldb, int c, int64_t e, int64_t e1) {
+  constexpr int G = 64 / KP;
+  double acc = 0.0;
+  for (; e + (kBatch - 1) * G < e1; e += kBatch * G) {
+    int32_t j[kBatch];
+    uint32_t q[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      j[u] = idx[e + u * G];
+      q[u] = cnt[e + u * G];
+    }
+    double s[kBatch], z[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      s[u] = CS ? b[j[u]] : 1.0;
+      z[u] = B[(int64_t)j[u] * ldb + c];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) acc = fma((double)q[u] * s[u], z[u], acc);
+  }
+  for (; e < e1; e += G) {
+    const int32_t j = idx[e];
+    acc = fma((double)cnt[e] * (CS ? b[j] : 1.0), B[(int64_t)j * ldb + c], acc);
+  }
+  return acc;
+}
+
+template <int KP, bool CS>
 __global__ void __launch_bounds__(kThreads) spmm_scaled_kernel(
     const int64_t* __restrict__ indptr, const int32_t* __restrict__ idx, const uint32_t* __restrict__ cnt,
     int64_t n, const double* __restrict__ a, const double* __restrict__ b, const double* __restrict__ B,
     int64_t ldb, int k, double* __restrict__ C, int64_t ldc) {
-  constexpr int G = 64 / KP;
   const int lane = threadIdx.x & 63;
   const int grp = lane / KP, col = lane % KP;
   const int64_t row = ((int64_t)blockIdx.x * kThreads + threadIdx.x) >> 6;
@@ -94,13 +132,7 @@ __global__ void __launch_bounds__(kThreads) spmm_scaled_kernel(
   for (int c0 = 0; c0 < k; c0 += KP) {
     const int c = c0 + col;
     const bool on = c < k;
-    double acc = 0.0;
-#pragma unroll 4
-    for (int64_t e = e0 + grp; e < e1; e += G) {
-      const int32_t j = idx[e];
-      const double x = (double)cnt[e] * (b ? b[j] : 1.0);
-      acc = fma(x, on ? B[(int64_t)j * ldb + c] : 0.0, acc);
-    }
+    double acc = scaled_row_sum<KP, CS>(idx, cnt, b, B, ldb, on ? c : k - 1, e0 + grp, e1);
 #pragma unroll
     for (int off = KP; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
     if (grp == 0 && on) C[row * ldc + c] = ai * acc;
@@ -269,9 +301,16 @@ int snapgpu_csr_spmm_scaled_f64(const int64_t* d_indptr, const int32_t* d_idx, c
   SG_ARG(n >= 0 && k >= 1 && ldb >= k && ldc >= k && d_B != nullptr && d_C != nullptr, "bad arguments");
   if (n == 0) return SNAPGPU_OK;
   hipStream_t st = as_stream(stream);
-#define SG_SPMM(KP)                                                                                            \
-  spmm_scaled_kernel<KP><<<waves_grid(n), kThreads, 0, st>>>(d_indptr, d_idx, d_cnt, n, d_row_scale, d_col_scale, \
-                                                             d_B, ldb, k, d_C, ldc)
+#define SG_SPMM(KP)                                                                                          \
+  do {                                                                                                       \
+    if (d_col_scale)                                                                                         \
+      spmm_scaled_kernel<KP, true><<<waves_grid(n), kThreads, 0, st>>>(d_indptr, d_idx, d_cnt, n, d_row_scale, \
+                                                                       d_col_scale, d_B, ldb, k, d_C, ldc);  \
+    else                                                                                                     \
+      spmm_scaled_kernel<KP, false><<<waves_grid(n), kThreads, 0, st>>>(d_indptr, d_idx, d_cnt, n,            \
+                                                                        d_row_scale, nullptr, d_B, ldb, k,   \
+                                                                        d_C, ldc);                           \
+  } while (0)
   if (k <= 1) SG_SPMM(1);
   else if (k <= 2) SG_SPMM(2);
   else if (k <= 4) SG_SPMM(4);

@@ -204,10 +204,12 @@ __global__ void __launch_bounds__(256) row_scale_kernel(const int64_t* indptr, c
   const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (row >= n) return;
   double acc = 0.0;
-  for (int64_t e = indptr[row] + lane; e < indptr[row + 1]; e += 64) {
-    const double x = val64<FV>(cnt, e) * w[idx[e]];
-    acc = fma(x, x, acc);
-  }
+  lane_entries<FV>(
+      idx, cnt, indptr[row] + lane, indptr[row + 1], acc, [&](int32_t j) { return w[j]; },
+      [](double& a, double v, double wj) {
+        const double x = v * wj;
+        a = fma(x, x, a);
+      });
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if (lane == 0) s[row] = acc > 0.0 ? 1.0 / sqrt(acc) : 0.0;
 }
@@ -221,8 +223,9 @@ __global__ void __launch_bounds__(256) colsum_kernel(const int64_t* colptr, cons
   const int64_t col = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (col >= m) return;
   double acc = 0.0;
-  for (int64_t e = colptr[col] + lane; e < colptr[col + 1]; e += 64)
-    acc = fma(val64<FV>(cnt, e), s[rowidx[e]], acc);
+  lane_entries<FV>(
+      rowidx, cnt, colptr[col] + lane, colptr[col + 1], acc, [&](int32_t i) { return s[i]; },
+      [](double& a, double v, double si) { a = fma(v, si, a); });
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if (lane == 0) c[col] = acc * w[col];
 }
@@ -239,10 +242,9 @@ __global__ void __launch_bounds__(256) degree_kernel(const int64_t* indptr, cons
   if (row >= n) return;
   const int64_t e0 = indptr[row], e1 = indptr[row + 1];
   double acc = 0.0;
-  for (int64_t e = e0 + lane; e < e1; e += 64) {
-    const int32_t j = idx[e];
-    acc = fma(val64<FV>(cnt, e) * w[j], c[j], acc);
-  }
+  lane_entries<FV>(
+      idx, cnt, e0 + lane, e1, acc, [&](int32_t j) { return make_double2(w[j], c[j]); },
+      [](double& a, double v, double2 wc) { a = fma(v * wc.x, wc.y, a); });
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if (lane == 0) {
     const double d = acc * s[row] - 1.0;
@@ -755,19 +757,23 @@ __global__ void __launch_bounds__(256) csc_spmv2_kernel(const int64_t* colptr, c
   const int lane = threadIdx.x & 63;
   const int64_t col = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (col >= m) return;
-  double a1 = 0.0, a2 = 0.0;
-  for (int64_t e = colptr[col] + lane; e < colptr[col + 1]; e += 64) {
-    double v = val64<FV>(cnt, e);
-    if (POW == 2) v *= v;
-    const int32_t i = rowidx[e];
-    if (x12 != nullptr) {
-      const double2 x = x12[i];
-      a1 = fma(v, x.x, a1);
-      a2 = fma(v, x.y, a2);
-    } else {
-      a1 = fma(v, x1[i], a1);
-    }
+  double2 a = make_double2(0.0, 0.0);
+  const int64_t e0 = colptr[col] + lane, e1 = colptr[col + 1];
+  if (x12 != nullptr) {
+    lane_entries<FV>(rowidx, cnt, e0, e1, a, [&](int32_t i) { return x12[i]; },
+                     [](double2& s, double v, double2 x) {
+                       if (POW == 2) v *= v;
+                       s.x = fma(v, x.x, s.x);
+                       s.y = fma(v, x.y, s.y);
+                     });
+  } else {
+    lane_entries<FV>(rowidx, cnt, e0, e1, a, [&](int32_t i) { return x1[i]; },
+                     [](double2& s, double v, double x) {
+                       if (POW == 2) v *= v;
+                       s.x = fma(v, x, s.x);
+                     });
   }
+  double a1 = a.x, a2 = a.y;
   for (int o = 32; o > 0; o >>= 1) {
     a1 += __shfl_xor(a1, o, 64);
     a2 += __shfl_xor(a2, o, 64);
@@ -787,10 +793,8 @@ __global__ void __launch_bounds__(256) csr_spmv_w2_kernel(const int64_t* indptr,
   const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (row >= n) return;
   double a = 0.0;
-  for (int64_t e = indptr[row] + lane; e < indptr[row + 1]; e += 64) {
-    const double2 q = w2x[idx[e]];
-    a = fma(val64<FV>(cnt, e) * q.x, q.y, a);
-  }
+  lane_entries<FV>(idx, cnt, indptr[row] + lane, indptr[row + 1], a, [&](int32_t j) { return w2x[j]; },
+                   [](double& s, double v, double2 q) { s = fma(v * q.x, q.y, s); });
   for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
   if (lane == 0) y[row] = a;
 }

@@ -127,19 +127,20 @@ def spectral_embedding_nystrom(X: DeviceCSR, n_components: int, sample_size: int
     E = torch.empty_like(U)
     _lib.check(L.snapgpu_nystrom_landmark_scale(U.data_ptr(), k, U.shape[0], k, op.dinv.data_ptr(), ev.data_ptr(),
                                                 E.data_ptr(), k, st))
-    # Z = seed~^T E = diag(w) X_seed^T diag(s_seed) E, on the landmark CSC
+    # Z = seed~^T E = diag(w) X_seed^T diag(s_seed) E, on the landmark CSC; the
+    # column scales are applied to the dense operand first (one gather per entry)
+    sE = (op.s[:, None] * E).contiguous()
     Z = torch.empty((m, k), dtype=torch.float64, device=dev)
     _lib.check(L.snapgpu_csr_spmm_scaled_f64(op.colptr.data_ptr(), op.rowidx.data_ptr(), op.cval.data_ptr(),
-                                             m, w.data_ptr(), op.s.data_ptr(), E.data_ptr(), k, k,
-                                             Z.data_ptr(), k, st))
+                                             m, w.data_ptr(), 0, sE.data_ptr(), k, k, Z.data_ptr(), k, st))
     # q = sample~ Z = diag(s) X diag(w) Z for all cells
     s_all = torch.empty(n, dtype=torch.float64, device=dev)
     _lib.check(L.snapgpu_csr_row_inv_norm(X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), n,
                                           w.data_ptr(), s_all.data_ptr(), st))
     Q = torch.empty((n, k), dtype=torch.float64, device=dev)
+    wZ = (w[:, None] * Z).contiguous()
     _lib.check(L.snapgpu_csr_spmm_scaled_f64(X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), n,
-                                             s_all.data_ptr(), w.data_ptr(), Z.data_ptr(), k, k,
-                                             Q.data_ptr(), k, st))
+                                             s_all.data_ptr(), 0, wZ.data_ptr(), k, k, Q.data_ptr(), k, st))
     wsb = int(L.snapgpu_nystrom_normalize_workspace_bytes(n, k, chunk_size))
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
     _lib.check(L.snapgpu_nystrom_normalize(Q.data_ptr(), k, n, k, ev.data_ptr(), int(chunk_size), ws.data_ptr(),

@@ -387,10 +387,12 @@ class SpectralOperator:
         U = U.to(torch.float64).contiguous()
         b = U.shape[1]
         w2 = (self.w * self.w).contiguous()
+        # r o U up front: the CSC product then gathers one row per entry, not also r
+        rU = (self.r64[:, None] * U).contiguous()
         Z = torch.empty((self.m, b), dtype=torch.float64, device=U.device)
         _lib.check(L.snapgpu_csr_spmm_scaled_f64(self.colptr.data_ptr(), self.rowidx.data_ptr(),
                                                  self.cval.data_ptr(), self.m, w2.data_ptr(),
-                                                 self.r64.data_ptr(), U.data_ptr(), b, b, Z.data_ptr(), b, st))
+                                                 0, rU.data_ptr(), b, b, Z.data_ptr(), b, st))
         Y = torch.empty_like(U)
         _lib.check(L.snapgpu_csr_spmm_scaled_f64(self.X.indptr.data_ptr(), self.X.indices.data_ptr(),
                                                  self.X.data.data_ptr(), self.n, self.r64.data_ptr(), 0,
