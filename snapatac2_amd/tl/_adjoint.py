@@ -20,7 +20,7 @@ from typing import NamedTuple
 import numpy as np
 
 from .. import _lib
-from ._lanczos import _Linalg, eigsh_lm
+from ._lanczos import eigsh_lm
 from ._operator import SpectralOperator
 
 
@@ -136,11 +136,10 @@ def _pair_f64(op: SpectralOperator, U, evals, k: int):
     mixing. Returns (V, (lambda_k, lambda_{k+1})); V unchanged (and the Lanczos
     eigenvalues) for float-valued matrices."""
     V = U[:, k - 1:k + 1].contiguous()
-    AV = op.apply_f64(V)
-    if AV is None:
+    H = op.gram_f64(V)  # V^T A V (2 x 2) from one FP64 sparse product
+    if H is None:
         return V, (float(evals[k - 1]), float(evals[k]))
-    la = _Linalg(op.torch, V.shape[0], 2, V.device)
-    H = la.tn(V, AV).cpu().numpy()  # the FP64 tall-skinny product (V^T AV, 2 x 2)
+    H = H.cpu().numpy()
     H = 0.5 * (H + H.T)
     lam, Q = np.linalg.eigh(H)
     order = np.argsort(-np.abs(lam), kind="stable")

@@ -399,6 +399,28 @@ class SpectralOperator:
                                                  Z.data_ptr(), b, b, Y.data_ptr(), b, st))
         return Y - self.dinv[:, None] * U
 
+    def gram_f64(self, U):
+        """U^T A U (b x b, FP64) with its sparse product in FP64. Since
+        A = R X W^2 X^T R - D^-1, U^T A U = P^T P - U^T D^-1 U with
+        P = W X^T (R U): one sparse product (the CSC, snapgpu_csr_spmm_scaled_f64
+        with row scales w) and two tall-skinny FP64 products, where apply_f64
+        takes two sparse products. b <= 32. None for float-valued matrices."""
+        if self.value_type != 0:
+            return None
+        from ._lanczos import _Linalg
+        torch = self.torch
+        L = _lib.lib()
+        U = U.to(torch.float64).contiguous()
+        b = U.shape[1]
+        rU = (self.r64[:, None] * U).contiguous()
+        P = torch.empty((self.m, b), dtype=torch.float64, device=U.device)
+        _lib.check(L.snapgpu_csr_spmm_scaled_f64(self.colptr.data_ptr(), self.rowidx.data_ptr(),
+                                                 self.cval.data_ptr(), self.m, self.w.data_ptr(), 0,
+                                                 rU.data_ptr(), b, b, P.data_ptr(), b, _lib.stream_ptr()))
+        PP = _Linalg(torch, self.m, b, U.device).tn(P, P).clone()
+        dU = (self.dinv[:, None] * U).contiguous()
+        return PP - _Linalg(torch, self.n, b, U.device).tn(U, dU)
+
     def rayleigh_f64(self, U):
         """Rayleigh quotients u^T A u of the columns of U with the FP64 products
         of apply_f64 (None for float-valued matrices)."""

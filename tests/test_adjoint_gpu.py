@@ -72,6 +72,10 @@ def test_apply_f64_matches_oracle(torch_gpu, b):
     assert _rel(got, ref) < 1e-12
     rq = op.rayleigh_f64(torch.from_numpy(V).cuda()).cpu().numpy()
     assert np.allclose(rq, (V * ref).sum(0), rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+    if b <= 32:  # V^T A V from the one-product form
+        H = op.gram_f64(torch.from_numpy(V).cuda()).cpu().numpy()
+        Href = V.T @ ref
+        assert np.abs(H - Href).max() <= 1e-11 * np.abs(Href).max()
 
 
 def test_vjp_finite_differences(torch_gpu):
