@@ -17,15 +17,18 @@ __global__ void __launch_bounds__(256) col_sum_kernel(const IdxT* __restrict__ i
 // A block owns kWinRows consecutive rows and sweeps the columns in windows of
 // kWinCols: per window, teams of 16 lanes walk each row from its cursor while
 // the column is inside the window (32 entries per step, kWinR rows per team at
-// once), add into a 64-bit LDS histogram, and the window's non-zero bins go to
-// global memory with one coalesced atomic each. Every entry is read once, and
-// global atomics drop from one per entry to one per (block, column). An entry
+// once), add into a 32-bit LDS histogram, and the window's non-zero bins go to
+// global memory with one coalesced 64-bit atomic each. Global atomics drop from
+// one per entry to one per (block, column). A step's entries past the window
+// are read again in the next window, so the window is as wide as LDS allows:
+// 32-bit bins give twice the columns of 64-bit ones. A bin can only wrap when
+// its block's values add up to 2^32 or more, which the block checks. An entry
 // whose column lies below the current window (a row out of order, a negative
 // index) and a row not exhausted after the last window (an index >= m) set
-// status bit 0; a block with 2^32 or more entries sets bit 1. The caller then
-// discards the output.
+// status bit 0; a block with 2^32 or more entries sets bit 1, and one whose
+// values add up to 2^32 or more bit 2. The caller then discards the output.
 // ---------------------------------------------------------------------------
-constexpr int kWinCols = 16384;   // 128 KB of u64 bins
+constexpr int kWinCols = 32768;   // 128 KB of u32 bins
 constexpr int kWinRows = 3968;    // rows per block: u32 cursors + row starts fit the rest of LDS
 constexpr int kWinThreads = 1024;
 constexpr int kWinTeams = kWinThreads / 16;
@@ -36,9 +39,10 @@ __global__ void __launch_bounds__(kWinThreads) col_window_kernel(
     const int64_t* __restrict__ indptr, int64_t n, const IdxT* __restrict__ idx,
     const uint32_t* __restrict__ val, int64_t m, unsigned long long* __restrict__ out,
     int32_t* __restrict__ status) {
-  __shared__ unsigned long long hist[kWinCols];
+  __shared__ uint32_t hist[kWinCols];
   __shared__ uint32_t cur[kWinRows];
   __shared__ uint32_t start[kWinRows + 1];
+  __shared__ unsigned long long block_sum;
   const int tid = threadIdx.x;
   const int64_t g0 = (int64_t)blockIdx.x * kWinRows;
   const int nr = (int)min<int64_t>(kWinRows, n - g0);
@@ -52,11 +56,13 @@ __global__ void __launch_bounds__(kWinThreads) col_window_kernel(
     start[i] = o;
     if (i < nr) cur[i] = o;
   }
+  if (tid == 0) block_sum = 0ull;
   const int team = tid >> 4, tl = tid & 15;
   const int tshift = tid & 48;  // this team's 16 bits in the wave's ballot
   bool bad = false;
+  unsigned long long mine = 0ull;  // values this thread added (the block's total bounds every bin)
   for (int64_t c0 = 0; c0 < m; c0 += kWinCols) {
-    for (int j = tid; j < kWinCols; j += kWinThreads) hist[j] = 0ull;
+    for (int j = tid; j < kWinCols; j += kWinThreads) hist[j] = 0u;
     __syncthreads();
     const int64_t c1 = c0 + kWinCols;
     for (int i0 = team; i0 < nr; i0 += kWinTeams * kWinR) {
@@ -95,12 +101,18 @@ __global__ void __launch_bounds__(kWinThreads) col_window_kernel(
           if (tl < take) {
             const int64_t c = (int64_t)ca[r];
             if (c < c0) bad = true;
-            else atomicAdd(&hist[c - c0], COUNT ? 1ull : (unsigned long long)va[r]);
+            else {
+              atomicAdd(&hist[c - c0], COUNT ? 1u : va[r]);
+              mine += COUNT ? 0u : va[r];
+            }
           }
           if (16 + tl < take) {
             const int64_t c = (int64_t)cb[r];
             if (c < c0) bad = true;
-            else atomicAdd(&hist[c - c0], COUNT ? 1ull : (unsigned long long)vb[r]);
+            else {
+              atomicAdd(&hist[c - c0], COUNT ? 1u : vb[r]);
+              mine += COUNT ? 0u : vb[r];
+            }
           }
           e[r] += take;
           more |= take == 32;
@@ -114,13 +126,18 @@ __global__ void __launch_bounds__(kWinThreads) col_window_kernel(
     }
     __syncthreads();
     for (int j = tid; j < kWinCols; j += kWinThreads) {
-      const unsigned long long h = hist[j];
-      if (h != 0ull && c0 + j < m) atomicAdd(&out[c0 + j], h);
+      const uint32_t h = hist[j];
+      if (h != 0u && c0 + j < m) atomicAdd(&out[c0 + j], (unsigned long long)h);
     }
     __syncthreads();
   }
   for (int i = tid; i < nr; i += kWinThreads) bad |= cur[i] != start[i + 1];
   if (bad) atomicOr(status, 1);
+  if (!COUNT) {  // counts cannot wrap: a block holds fewer than 2^32 entries
+    atomicAdd(&block_sum, mine);
+    __syncthreads();
+    if (tid == 0 && block_sum >= (1ull << 32)) atomicOr(status, 4);
+  }
 }
 
 template <typename IdxT, bool COUNT>

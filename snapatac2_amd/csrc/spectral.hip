@@ -111,6 +111,104 @@ __global__ void __launch_bounds__(256) select_fill_kernel(const int64_t* indptr,
   }
 }
 
+// Both selection passes for 32-bit column indices, with the table in LDS. With
+// one table lookup per entry straight from L2, every lane's lookup is its own
+// L2 request (a row's neighbouring entries lie hundreds of columns apart), and
+// the request rate, not HBM, sets the pace. Here a block owns kSelRows rows and
+// sweeps the columns in windows of kSelWinWords table words (262,144 columns):
+// it stages the window's slice of the table in LDS, then each wave walks its
+// rows from their cursors while the columns stay below the window's end, 64
+// entries per step, and looks the columns up in LDS. Entries are visited once,
+// in row order, so counts and output equal the per-entry kernels' for any input:
+// a column below the window (a row out of order, or negative) is looked up in
+// the global table, and the last window takes every remaining entry (columns
+// past the table select nothing).
+constexpr int kSelWinWords = 8192;  // 64 KB of table words
+constexpr int kSelRows = 512;
+constexpr int kSelThreads = 1024;
+
+template <bool FILL>
+__global__ void __launch_bounds__(kSelThreads) select_window_kernel(
+    const int64_t* __restrict__ indptr, const int32_t* __restrict__ idx, const uint32_t* __restrict__ val,
+    int64_t n, const uint32_t* __restrict__ sel, int64_t words, int64_t* __restrict__ row_nnz,
+    const int64_t* __restrict__ new_indptr, int32_t* __restrict__ new_idx, uint32_t* __restrict__ new_val) {
+  __shared__ uint2 tab[kSelWinWords];
+  __shared__ int64_t cur[kSelRows];  // next entry of the row
+  __shared__ int64_t end[kSelRows];
+  __shared__ int64_t out[kSelRows];  // count: selected so far; fill: next output slot
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int64_t r0 = (int64_t)blockIdx.x * kSelRows;
+  const int nr = (int)min<int64_t>(kSelRows, n - r0);
+  for (int i = tid; i < nr; i += kSelThreads) {
+    cur[i] = indptr[r0 + i];
+    end[i] = indptr[r0 + i + 1];
+    out[i] = FILL ? new_indptr[r0 + i] : 0;
+  }
+  const int64_t nwin = (words + kSelWinWords - 1) / kSelWinWords;
+  for (int64_t w = 0; w < nwin; ++w) {
+    const int64_t w0 = w * kSelWinWords;
+    const int nw = (int)min<int64_t>(kSelWinWords, words - w0);
+    __syncthreads();  // the previous window's walks are done with tab (and the cursors are set)
+    for (int j = tid; j < nw; j += kSelThreads) tab[j] = *reinterpret_cast<const uint2*>(sel + 2 * (w0 + j));
+    __syncthreads();
+    const int64_t c_lo = w0 * 32;
+    const int64_t c_hi = w + 1 == nwin ? INT64_MAX : (w0 + nw) * 32;
+    for (int i = wid; i < nr; i += kSelThreads / 64) {
+      int64_t e = cur[i], o = out[i];
+      const int64_t e1 = end[i];
+      int32_t c = 0;
+      uint32_t v = 0;
+      if (e + lane < e1) {
+        c = idx[e + lane];
+        if (FILL) v = val[e + lane];
+      }
+      while (e < e1) {
+        const int64_t ee = e + lane;
+        // the next step's entries are loaded before this step is used (values
+        // too: at a quarter of the columns selected nearly every 64-byte
+        // segment of values holds a selected entry, so they cost no traffic)
+        int32_t cn = 0;
+        uint32_t vn = 0;
+        if (ee + 64 < e1) {
+          cn = idx[ee + 64];
+          if (FILL) vn = val[ee + 64];
+        }
+        const uint64_t tk = ballot64(ee < e1 && (int64_t)c < c_hi);
+        const int take = ~tk == 0 ? 64 : __builtin_ctzll(~tk);  // leading run of entries in the window
+        int32_t nc = -1;
+        if (lane < take) {
+          uint2 s;
+          const int64_t wd = (int64_t)(c >> 5);
+          if ((int64_t)c < c_lo) s = sel_word(sel, words, c);
+          else if (wd < w0 + nw) s = tab[wd - w0];
+          else s = make_uint2(0u, 0u);  // past the table (last window only)
+          const uint32_t bit = (uint32_t)c & 31u;
+          if ((s.x >> bit) & 1u) nc = (int32_t)(s.y + __popc(s.x & ((1u << bit) - 1u)));
+        }
+        const uint64_t m = ballot64(nc >= 0);
+        if (FILL && nc >= 0) {
+          const int64_t p = o + popc64(m & lanemask_lt(lane));
+          new_idx[p] = nc;
+          new_val[p] = v;
+        }
+        o += popc64(m);
+        e += take;
+        if (take < 64) break;  // the row's end, or an entry at or past the window's end
+        c = cn;
+        v = vn;
+      }
+      if (lane == 0) {
+        cur[i] = e;
+        out[i] = o;
+      }
+    }
+  }
+  if (!FILL) {
+    __syncthreads();
+    for (int i = tid; i < nr; i += kSelThreads) row_nnz[r0 + i] = out[i];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Stable CSR -> CSC transpose (rows stay ascending inside each column)
 // ---------------------------------------------------------------------------
@@ -1000,8 +1098,8 @@ int snapgpu_csr_select_count(const int64_t* d_indptr, const void* d_idx, int32_t
   const unsigned g = (unsigned)ceil_div(n, 4);
   const int64_t words = ceil_div(m, 32);
   if (index_bits == 32)
-    select_count_kernel<int32_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int32_t*)d_idx, n, d_sel,
-                                                                   words, d_row_nnz);
+    select_window_kernel<false><<<(unsigned)ceil_div(n, kSelRows), kSelThreads, 0, as_stream(stream)>>>(
+        d_indptr, (const int32_t*)d_idx, nullptr, n, d_sel, words, d_row_nnz, nullptr, nullptr, nullptr);
   else
     select_count_kernel<int64_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int64_t*)d_idx, n, d_sel,
                                                                    words, d_row_nnz);
@@ -1019,9 +1117,8 @@ int snapgpu_csr_select_fill(const int64_t* d_indptr, const void* d_idx, int32_t 
   const unsigned g = (unsigned)ceil_div(n, 4);
   const int64_t words = ceil_div(m, 32);
   if (index_bits == 32)
-    select_fill_kernel<int32_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int32_t*)d_idx, d_val, n,
-                                                                  d_sel, words, d_new_indptr, d_new_idx,
-                                                                  d_new_val);
+    select_window_kernel<true><<<(unsigned)ceil_div(n, kSelRows), kSelThreads, 0, as_stream(stream)>>>(
+        d_indptr, (const int32_t*)d_idx, d_val, n, d_sel, words, nullptr, d_new_indptr, d_new_idx, d_new_val);
   else
     select_fill_kernel<int64_t><<<g, 256, 0, as_stream(stream)>>>(d_indptr, (const int64_t*)d_idx, d_val, n,
                                                                   d_sel, words, d_new_indptr, d_new_idx,
