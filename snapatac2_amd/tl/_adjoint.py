@@ -97,8 +97,8 @@ def spectral_gap(op: SpectralOperator, n_components: int = 30, seed: int = 0, v0
     return gap, evals, U
 
 
-def _gap_stats(op: SpectralOperator, n_components: int, seed: int, v0=None):
-    evals, U, st = eigsh_lm(op, n_components + 1, seed=seed, v0=v0)
+def _gap_stats(op: SpectralOperator, n_components: int, seed: int, v0=None, tol: float = 1e-9):
+    evals, U, st = eigsh_lm(op, n_components + 1, seed=seed, v0=v0, tol=tol)
     k = n_components
     return float(abs(evals[k - 1]) - abs(evals[k])), evals, U, st
 
@@ -148,9 +148,9 @@ def _pair_f64(op: SpectralOperator, U, evals, k: int):
     return V, (float(lam[0]), float(lam[1]))
 
 
-def _gap_grad_stats(op: SpectralOperator, n_components: int, seed: int, v0=None):
+def _gap_grad_stats(op: SpectralOperator, n_components: int, seed: int, v0=None, tol: float = 1e-9):
     torch = op.torch
-    gap, evals, U, st = _gap_stats(op, n_components, seed, v0)
+    gap, evals, U, st = _gap_stats(op, n_components, seed, v0, tol)
     k = n_components
     V, (l0, l1) = _pair_f64(op, U, evals, k)
     sg = torch.tensor([np.sign(l0) or 1.0, -(np.sign(l1) or 1.0)], dtype=torch.float64, device=V.device)
@@ -159,7 +159,8 @@ def _gap_grad_stats(op: SpectralOperator, n_components: int, seed: int, v0=None)
 
 
 def tune_feature_weights(op: SpectralOperator, n_components: int = 30, steps: int = 10,
-                         step_size: float = 0.05, seed: int = 0, min_step: float = 1e-4):
+                         step_size: float = 0.05, seed: int = 0, min_step: float = 1e-4,
+                         tol: float = 1e-9, grad_tol: float = 1e-7):
     """Gradient ascent on the eigengap |lambda_k| - |lambda_{k+1}| over w >= 0.
 
     Each step moves w by step_size * mean(w) * g / max|g| and is kept only if
@@ -172,11 +173,21 @@ def tune_feature_weights(op: SpectralOperator, n_components: int = 30, steps: in
     and the n_components + 1 eigenpairs of largest magnitude for the returned
     weights, so a caller can embed with them without another eigensolve. `op` is
     left prepared for the returned weights. Each trial eigensolve starts from
-    the eigenvectors of the current weights."""
+    the eigenvectors of the current weights.
+
+    Eigensolves whose pairs feed a gradient stop at the relative residual
+    grad_tol, the last trial's at tol: the acceptance margin for a residual r is
+    2 r^2 / gap (6e-9 for r = 1e-7 against the gap of 1.5e-6 at a million
+    cells), far below any gap change a step is kept for, so the rounds from
+    grad_tol to tol would not change a decision. The eigenpairs returned are
+    always at tol: when they come from a grad_tol solve (no trial kept after
+    it), a solve at tol started from them replaces them."""
     torch = op.torch
     k = n_components
     w = op.w.clone()
-    _, g, evals, U, st, pair = _gap_grad_stats(op, n_components, seed)
+    first_tol = grad_tol if steps > 0 else tol
+    _, g, evals, U, st, pair = _gap_grad_stats(op, n_components, seed, tol=first_tol)
+    precise = first_tol <= tol  # whether evals, U were solved to tol
     gap, unc = _checked_gap(op, evals, U, st, k, pair)
     history = [(gap, 0.0)]
     for it in range(steps):
@@ -191,19 +202,24 @@ def tune_feature_weights(op: SpectralOperator, n_components: int = 30, steps: in
         try:
             op.set_weights(trial)
             if last:  # no further step: the gradient at the trial point is not needed
-                _, new_evals, new_U, new_st = _gap_stats(op, n_components, seed, v0=U)
-                new_g, new_pair = None, None
+                _, new_evals, new_U, new_st = _gap_stats(op, n_components, seed, v0=U, tol=tol)
+                new_g, new_pair, new_precise = None, None, True
             else:
-                _, new_g, new_evals, new_U, new_st, new_pair = _gap_grad_stats(op, n_components, seed, v0=U)
+                _, new_g, new_evals, new_U, new_st, new_pair = _gap_grad_stats(op, n_components, seed, v0=U,
+                                                                               tol=grad_tol)
+                new_precise = grad_tol <= tol
             new_gap, new_unc = _checked_gap(op, new_evals, new_U, new_st, k, new_pair)
         except _lib.PanicException:
             new_gap, new_unc = -np.inf, 0.0
         if new_gap - new_unc > gap + unc:
             w, gap, unc, g, evals, U = trial, new_gap, new_unc, new_g, new_evals, new_U
+            precise = new_precise
             history.append((gap, step_size))
             step_size *= 1.5
         else:
             step_size *= 0.5
     if not torch.equal(op.w, w):
         op.set_weights(w)
+    if not precise:
+        evals, U, _ = eigsh_lm(op, n_components + 1, seed=seed, v0=U, tol=tol)
     return TuneResult(w, history, evals, U)

@@ -181,6 +181,37 @@ def test_eigengap_ascent(torch_gpu):
     assert (w >= 0).all()
 
 
+def test_ascent_tolerances(torch_gpu, monkeypatch):
+    """Eigensolves that only feed a gradient stop at grad_tol (1e-7); the
+    eigenpairs returned come from a solve at tol (1e-9): the last trial's when
+    it is kept, else a solve at tol started from the kept pairs. Both agree with
+    a fresh solve at tol (to the FP32 products' rounding)."""
+    torch = torch_gpu
+    from snapatac2_amd.tl import _adjoint as A
+    X = _counts(3000, 1500, 0.02, seed=71)  # above 1024 cells: block Lanczos
+    calls = []
+    real = A.eigsh_lm
+
+    def spy(op, k, **kw):
+        calls.append((kw.get("tol"), kw.get("v0") is not None))
+        return real(op, k, **kw)
+
+    monkeypatch.setattr(A, "eigsh_lm", spy)
+    k = 6
+    for step_size, kept in ((0.01, True), (1e-13, False)):
+        op = _op(torch, X, None)
+        w0 = op.w.clone()
+        calls.clear()
+        w, hist, ev, U = A.tune_feature_weights(op, k, steps=1, step_size=step_size, min_step=0.0)
+        assert (len(hist) == 2) == kept
+        assert [t for t, _ in calls] == ([1e-7, 1e-9] if kept else [1e-7, 1e-9, 1e-9])
+        assert calls[-1][1]  # the last solve starts from earlier eigenvectors
+        if not kept:
+            assert torch.equal(w, w0) and torch.equal(op.w, w0)
+        ev2, _, _ = real(op, k + 1, seed=1, tol=1e-9)
+        np.testing.assert_allclose(ev, ev2, rtol=0, atol=1e-6 * abs(ev2[0]))
+
+
 def test_vjp_generator_200k_binary(torch_gpu):
     """200,000 generator cells, 500 bp tiles, top 500,000 features, binarised;
     V (FP32 values) and G 200,000 x 30. References in FP64 through torch's own
