@@ -110,8 +110,7 @@ template <int CAP>
 struct WaveLds {
   uint32_t pbin[CAP];        // pending bins (sorted, distinct)
   uint32_t pcnt[CAP];
-  uint32_t nbin[128];        // run-length-encoded new events of one round
-  uint32_t ncnt[128];
+  uint32_t nbin[128];        // run starts (bins) of one round's new events
   uint32_t mbin[CAP + 128];  // merge buffer
   uint32_t mcnt[CAP + 128];
   uint64_t fscan[64];        // fragment strategy: inclusive scan of events per lane
@@ -274,13 +273,14 @@ __device__ void merge_round(CellState<CAP, FILL, IdxT>& s, const Params& p, uint
   const int nN = n0 + popc64(S1);
   const uint64_t lt = lanemask_lt(lane), gt = lanemask_gt(lane);
   int idx0 = popc64(S0 & lt), idx1 = n0 + popc64(S1 & lt);
+  // run lengths (the counts) only matter when writing; counting needs the runs alone
   uint32_t cnt0 = 0, cnt1 = 0;
-  if (st0) {
+  if (FILL && st0) {
     uint64_t m = S0 & gt;
     int nxt = m ? lowbit64(m) : (S1 ? 64 + lowbit64(S1) : nv);
     cnt0 = (uint32_t)(nxt - lane);
   }
-  if (st1) {
+  if (FILL && st1) {
     uint64_t m = S1 & gt;
     int nxt = m ? 64 + lowbit64(m) : nv;
     cnt1 = (uint32_t)(nxt - (64 + lane));
@@ -297,8 +297,8 @@ __device__ void merge_round(CellState<CAP, FILL, IdxT>& s, const Params& p, uint
     // pending: starts with key >= T, ranked after the finals
     const int np = nN - nf;
     if (np > CAP) { s.overflow = true; return; }
-    if (st0 && !f0) { L->pbin[idx0 - nf] = k[0]; L->pcnt[idx0 - nf] = cnt0; }
-    if (st1 && !f1) { L->pbin[idx1 - nf] = k[1]; L->pcnt[idx1 - nf] = cnt1; }
+    if (st0 && !f0) { L->pbin[idx0 - nf] = k[0]; if (FILL) L->pcnt[idx0 - nf] = cnt0; }
+    if (st1 && !f1) { L->pbin[idx1 - nf] = k[1]; if (FILL) L->pcnt[idx1 - nf] = cnt1; }
     s.emitted += nf;
     s.np = np;
     wave_sync();
@@ -306,22 +306,25 @@ __device__ void merge_round(CellState<CAP, FILL, IdxT>& s, const Params& p, uint
   }
 
   // General path: merge runs N with pending P by rank into M.
-  if (st0) { L->nbin[idx0] = k[0]; L->ncnt[idx0] = cnt0; }
-  if (st1) { L->nbin[idx1] = k[1]; L->ncnt[idx1] = cnt1; }
+  if (st0) L->nbin[idx0] = k[0];
+  if (st1) L->nbin[idx1] = k[1];
   wave_sync();
   const int np = s.np;
   if (st0) {
     int r = idx0 + lds_rank(L->pbin, np, k[0], false);
-    L->mbin[r] = k[0]; L->mcnt[r] = cnt0;
+    L->mbin[r] = k[0];
+    if (FILL) L->mcnt[r] = cnt0;
   }
   if (st1) {
     int r = idx1 + lds_rank(L->pbin, np, k[1], false);
-    L->mbin[r] = k[1]; L->mcnt[r] = cnt1;
+    L->mbin[r] = k[1];
+    if (FILL) L->mcnt[r] = cnt1;
   }
   for (int j = lane; j < np; j += 64) {
     uint32_t b = L->pbin[j];
     int r = j + lds_rank(L->nbin, nN, b, true);
-    L->mbin[r] = b; L->mcnt[r] = L->pcnt[j];
+    L->mbin[r] = b;
+    if (FILL) L->mcnt[r] = L->pcnt[j];
   }
   wave_sync();
   // Equal bins appear as adjacent pairs (pending first); fold them and split at T.
@@ -335,15 +338,20 @@ __device__ void merge_round(CellState<CAP, FILL, IdxT>& s, const Params& p, uint
       b = L->mbin[sidx];
       st = sidx == 0 || L->mbin[sidx - 1] != b;
       if (st) {
-        c = L->mcnt[sidx];
-        if (sidx + 1 < nM && L->mbin[sidx + 1] == b) c += L->mcnt[sidx + 1];
+        if (FILL) {
+          c = L->mcnt[sidx];
+          if (sidx + 1 < nM && L->mbin[sidx + 1] == b) c += L->mcnt[sidx + 1];
+        }
         fin = b < T;
       }
     }
     const uint64_t F = ballot64(st && fin), P = ballot64(st && !fin);
     if (st && fin) emit(s, p, s.emitted + run_final + popc64(F & lt), b, c);
     const int pr = run_pend + popc64(P & lt);
-    if (st && !fin && pr < CAP) { L->pbin[pr] = b; L->pcnt[pr] = c; }
+    if (st && !fin && pr < CAP) {
+      L->pbin[pr] = b;
+      if (FILL) L->pcnt[pr] = c;
+    }
     run_final += popc64(F);
     run_pend += popc64(P);
   }
