@@ -356,6 +356,19 @@ int snapgpu_spectral_apply_panel(const snapgpu_panel_layout* csr, const snapgpu_
                                  const double* d_V, int64_t ldv, int32_t b, double* d_Y,
                                  int64_t ldy, void* d_ws, int64_t ws_bytes, void* stream);
 
+/* out = diag(row_scale) X D in FP64 over one panel layout, for a dense operand D
+ * of dense_rows x 2 FP64 (row-major, 16-byte rows); X's values are squared when
+ * square_values != 0; row_scale may be NULL (1). Each (row, panel) segment is
+ * summed in entry order, then a row's segments in panel order, so results are
+ * bitwise reproducible. out: n_rows x 2. No counterpart in the reference: the
+ * FP64 products of this library's eigenpair checks and weight adjoint (the
+ * CSR form is snapgpu_csr_spmm_scaled_f64), gathering from an L2-sized slice
+ * of D at a time. Workspace: snapgpu_panel_spmm_f64_workspace_bytes(n_pairs). */
+int64_t snapgpu_panel_spmm_f64_workspace_bytes(int64_t n_pairs);
+int snapgpu_panel_spmm_f64(const snapgpu_panel_layout* layout, const double* d_dense, int32_t square_values,
+                           const double* d_row_scale, double* d_out, void* d_ws, int64_t ws_bytes,
+                           void* stream);
+
 /* Window layout (the same products as snapgpu_spectral_apply, b <= 16, counts
  * below 256; replaces the matvec of the reference's eigensolver call,
  * snapatac2-core/src/embedding.rs:155-171 via the operator of :269-326). One

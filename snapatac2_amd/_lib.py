@@ -66,6 +66,8 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_spectral_apply_panel_workspace_bytes": (_i64, [_i64, _i64, _i64, _i64]),
     "snapgpu_spectral_apply_panel": (C.c_int, [_p, _p, _p, _p, _p, _p, _i64, _i32, _p, _i64, _p,
                                                _i64, _p]),
+    "snapgpu_panel_spmm_f64_workspace_bytes": (_i64, [_i64]),
+    "snapgpu_panel_spmm_f64": (C.c_int, [_p, _p, _i32, _p, _p, _p, _i64, _p]),
     "snapgpu_block_panel_rows": (_i32, []),
     "snapgpu_block_plan_workspace_bytes": (_i64, [_i64]),
     "snapgpu_block_plan": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _i32, _i64, _p, _i64, C.POINTER(_i64),

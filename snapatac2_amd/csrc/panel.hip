@@ -488,6 +488,130 @@ static void launch_product(const snapgpu_panel_layout& L, const float* dense, fl
       L.row_pair_ptr, L.row_pair_ids, L.n_rows, partial, scale, out32, V, ldv, dinv, Y, ldy, b);
 }
 
+// ---------------------------------------------------------------------------
+// FP64 products over the same layout: a dense operand of two FP64 columns (a
+// 16-byte row per stored entry, so one L2 request like the FP32 products') and
+// FP64 pair sums. Groups of two lanes, one column each, stream the group-chunks
+// exactly as product_kernel does (the same pairs, each summed in entry order);
+// SQ multiplies by the squared value. reduce_f64_kernel adds a row's pairs in
+// panel order and applies an optional row scale.
+// ---------------------------------------------------------------------------
+template <int MODE, bool FV, bool SQ>
+__global__ void __launch_bounds__(256) product_f64_kernel(const uint32_t* __restrict__ ent,
+                                                          const uint32_t* __restrict__ val,
+                                                          const int64_t* __restrict__ cstart,
+                                                          const int32_t* __restrict__ cpair,
+                                                          const int32_t* __restrict__ cpanel, int64_t nchunks,
+                                                          const double* __restrict__ dense,
+                                                          double* __restrict__ partial) {
+  constexpr int BW = 2;
+  constexpr int G = 64 / BW;
+  constexpr int NB = 4 * BW;  // entries per group batch
+  constexpr bool PACK = MODE != 0;
+  constexpr uint32_t kFirst = MODE == 1 ? 0x8000u : 0x80000000u;
+  const int lane = threadIdx.x & 63;
+  const int g = lane / BW;
+  const int c = lane % BW;
+  const int64_t nb = gridDim.x, q8 = nb / 8, r8 = nb % 8;
+  const int64_t xcd = blockIdx.x % 8;
+  const int64_t blk = xcd * q8 + min(xcd, r8) + blockIdx.x / 8;
+  const int64_t chunk = ((blk * blockDim.x + threadIdx.x) >> 6) * G + g;
+  int64_t pos = 0, end = 0;
+  int64_t k = 0;
+  const double* db = dense;
+  if (chunk < nchunks) {
+    pos = cstart[chunk];
+    end = cstart[chunk + 1];
+    k = cpair[chunk];
+    db = dense + (int64_t)cpanel[chunk] * kPanel * BW;
+  }
+  bool have = false;
+  double acc = 0.0;
+  while (__ballot(pos < end) != 0) {
+    uint32_t e[4], v[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int64_t q = pos + s * BW + c;
+      e[s] = q < end ? __builtin_nontemporal_load(ent + q) : 0u;
+      v[s] = (!PACK && q < end) ? __builtin_nontemporal_load(val + q) : 0u;
+    }
+    const int nvalid = (int)min((int64_t)NB, max((int64_t)0, end - pos));
+    uint32_t ej[NB], vj[NB];
+    double z[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      ej[j] = (uint32_t)__shfl((int)e[j / BW], g * BW + j % BW, 64);
+      vj[j] = PACK ? 0u : (uint32_t)__shfl((int)v[j / BW], g * BW + j % BW, 64);
+      z[j] = j < nvalid ? db[(int64_t)entry_column<MODE>(ej[j]) * BW + c] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (ej[j] & kFirst) {
+        if (have) partial[k++ * BW + c] = acc;
+        have = true;
+        acc = 0.0;
+      }
+      // exact for every stored value (a count of 2^24 or more is not a float)
+      double x = MODE == 2   ? (double)((ej[j] >> 24) & 0x7Fu)
+                 : MODE == 1 ? (double)(ej[j] >> 16)
+                 : FV        ? (double)__uint_as_float(vj[j])
+                             : (double)vj[j];
+      if (SQ) x *= x;
+      acc = fma(x, z[j], acc);
+    }
+    pos += NB;
+  }
+  if (have) partial[k * BW + c] = acc;
+}
+
+__global__ void __launch_bounds__(256) reduce_f64_kernel(const int64_t* __restrict__ rp_ptr,
+                                                         const int32_t* __restrict__ rp_ids, int64_t nrows,
+                                                         const double* __restrict__ partial,
+                                                         const double* __restrict__ scale,
+                                                         double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t row = t >> 1;
+  const int c = (int)(t & 1);
+  if (row >= nrows) return;
+  const int64_t i0 = rp_ptr[row], i1 = rp_ptr[row + 1];
+  double acc = 0.0;
+  int64_t i = i0;
+  for (; i + 4 <= i1; i += 4) {
+    const int32_t p0 = rp_ids[i], p1 = rp_ids[i + 1], p2 = rp_ids[i + 2], p3 = rp_ids[i + 3];
+    const double a0 = partial[(int64_t)p0 * 2 + c], a1 = partial[(int64_t)p1 * 2 + c];
+    const double a2 = partial[(int64_t)p2 * 2 + c], a3 = partial[(int64_t)p3 * 2 + c];
+    acc += a0;
+    acc += a1;
+    acc += a2;
+    acc += a3;
+  }
+  for (; i < i1; ++i) acc += partial[(int64_t)rp_ids[i] * 2 + c];
+  out[row * 2 + c] = scale ? scale[row] * acc : acc;
+}
+
+template <bool SQ>
+static void launch_product_f64(const snapgpu_panel_layout& L, const double* dense, double* partial,
+                               const double* scale, double* out, hipStream_t st) {
+  const int64_t waves = ceil_div(L.n_chunks, 32);
+  if (waves > 0) {
+    const unsigned grid = (unsigned)ceil_div(waves, 4);
+    if (L.packed == 2)
+      product_f64_kernel<2, false, SQ><<<grid, 256, 0, st>>>(L.ent, L.val, L.chunk_start, L.chunk_pair,
+                                                             L.chunk_panel, L.n_chunks, dense, partial);
+    else if (L.packed == 1)
+      product_f64_kernel<1, false, SQ><<<grid, 256, 0, st>>>(L.ent, L.val, L.chunk_start, L.chunk_pair,
+                                                             L.chunk_panel, L.n_chunks, dense, partial);
+    else if (L.value_type == 1)
+      product_f64_kernel<0, true, SQ><<<grid, 256, 0, st>>>(L.ent, L.val, L.chunk_start, L.chunk_pair,
+                                                            L.chunk_panel, L.n_chunks, dense, partial);
+    else
+      product_f64_kernel<0, false, SQ><<<grid, 256, 0, st>>>(L.ent, L.val, L.chunk_start, L.chunk_pair,
+                                                             L.chunk_panel, L.n_chunks, dense, partial);
+  }
+  reduce_f64_kernel<<<(unsigned)ceil_div(L.n_rows * 2, 256), 256, 0, st>>>(L.row_pair_ptr, L.row_pair_ids,
+                                                                          L.n_rows, partial, scale, out);
+}
+
 // Build workspace: seg, F/S, L/E, scan workspace, pstart, small tables.
 struct BuildWs {
   uint32_t* seg;
@@ -685,6 +809,25 @@ int snapgpu_spectral_apply_panel(const snapgpu_panel_layout* csr, const snapgpu_
     SG_APPLY_PANEL(32)
   }
 #undef SG_APPLY_PANEL
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
+int64_t snapgpu_panel_spmm_f64_workspace_bytes(int64_t n_pairs) { return align256(std::max<int64_t>(n_pairs, 1) * 16) + 256; }
+
+int snapgpu_panel_spmm_f64(const snapgpu_panel_layout* L, const double* d_dense, int32_t square_values,
+                           const double* d_row_scale, double* d_out, void* d_ws, int64_t ws_bytes, void* stream) {
+  clear_error();
+  SG_ARG(L != nullptr && d_dense != nullptr && d_out != nullptr && d_ws != nullptr, "null pointer");
+  SG_ARG(L->packed >= 0 && L->packed <= 2, "bad packing");
+  SG_ARG(ws_bytes >= snapgpu_panel_spmm_f64_workspace_bytes(L->n_pairs), "workspace too small");
+  if (L->n_rows == 0) return SNAPGPU_OK;
+  hipStream_t st = as_stream(stream);
+  double* part = reinterpret_cast<double*>(((uintptr_t)d_ws + 255) & ~(uintptr_t)255);
+  if (square_values)
+    launch_product_f64<true>(*L, d_dense, part, d_row_scale, d_out, st);
+  else
+    launch_product_f64<false>(*L, d_dense, part, d_row_scale, d_out, st);
   SG_LAUNCH_CHECK();
   return SNAPGPU_OK;
 }

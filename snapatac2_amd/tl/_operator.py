@@ -402,9 +402,11 @@ class SpectralOperator:
     def gram_f64(self, U):
         """U^T A U (b x b, FP64) with its sparse product in FP64. Since
         A = R X W^2 X^T R - D^-1, U^T A U = P^T P - U^T D^-1 U with
-        P = W X^T (R U): one sparse product (the CSC, snapgpu_csr_spmm_scaled_f64
-        with row scales w) and two tall-skinny FP64 products, where apply_f64
-        takes two sparse products. b <= 32. None for float-valued matrices."""
+        P = W X^T (R U): one sparse product (the CSC: over its panel layout,
+        snapgpu_panel_spmm_f64, when the operator has one, else
+        snapgpu_csr_spmm_scaled_f64, with row scales w) and two tall-skinny FP64
+        products, where apply_f64 takes two sparse products. b <= 32. None for
+        float-valued matrices."""
         if self.value_type != 0:
             return None
         from ._lanczos import _Linalg
@@ -414,9 +416,25 @@ class SpectralOperator:
         b = U.shape[1]
         rU = (self.r64[:, None] * U).contiguous()
         P = torch.empty((self.m, b), dtype=torch.float64, device=U.device)
-        _lib.check(L.snapgpu_csr_spmm_scaled_f64(self.colptr.data_ptr(), self.rowidx.data_ptr(),
-                                                 self.cval.data_ptr(), self.m, self.w.data_ptr(), 0,
-                                                 rU.data_ptr(), b, b, P.data_ptr(), b, _lib.stream_ptr()))
+        if self.layout == "panel":
+            # two columns at a time over the CSC panel layout (L2-sized slices of rU)
+            ws_b = int(L.snapgpu_panel_spmm_f64_workspace_bytes(self.panel_csc.n_pairs))
+            if self._pws_b >= ws_b:
+                ws, ws_b = self._pws, self._pws_b
+            else:
+                ws = torch.empty(ws_b, dtype=torch.uint8, device=U.device)
+            if b % 2:
+                rU = torch.nn.functional.pad(rU, (0, 1))
+            P2 = torch.empty((self.m, 2), dtype=torch.float64, device=U.device)
+            for c0 in range(0, b, 2):
+                D = rU[:, c0:c0 + 2].contiguous()
+                _lib.check(L.snapgpu_panel_spmm_f64(C.byref(self.panel_csc), D.data_ptr(), 0, self.w.data_ptr(),
+                                                    P2.data_ptr(), ws.data_ptr(), ws_b, _lib.stream_ptr()))
+                P[:, c0:c0 + 2] = P2[:, :min(2, b - c0)]
+        else:
+            _lib.check(L.snapgpu_csr_spmm_scaled_f64(self.colptr.data_ptr(), self.rowidx.data_ptr(),
+                                                     self.cval.data_ptr(), self.m, self.w.data_ptr(), 0,
+                                                     rU.data_ptr(), b, b, P.data_ptr(), b, _lib.stream_ptr()))
         PP = _Linalg(torch, self.m, b, U.device).tn(P, P).clone()
         dU = (self.dinv[:, None] * U).contiguous()
         return PP - _Linalg(torch, self.n, b, U.device).tn(U, dU)

@@ -78,6 +78,26 @@ def test_apply_f64_matches_oracle(torch_gpu, b):
         assert np.abs(H - Href).max() <= 1e-11 * np.abs(Href).max()
 
 
+@pytest.mark.parametrize("b", [2, 3])
+def test_gram_f64_panel_layout(torch_gpu, b):
+    """V^T A V over the CSC panel layout (several panels of cells, FP64 sums)
+    against the FP64 oracle, and against the CSR-form product."""
+    torch = torch_gpu
+    from snapatac2_amd.tl._operator import SpectralOperator, _to_device_csr
+    X = _counts(40000, 36000, 0.004, seed=31)
+    w = _weights(X, 31)
+    Xd = _to_device_csr(X, torch)
+    op = SpectralOperator(Xd, w, use_panels=True)
+    op_csr = SpectralOperator(Xd, w, use_panels=False)
+    assert op.layout == "panel" and op_csr.layout == "csr"
+    V = np.random.default_rng(9).standard_normal((40000, b))
+    Vt = torch.from_numpy(V).cuda()
+    H = op.gram_f64(Vt).cpu().numpy()
+    Href = V.T @ adj.apply(X, w, V)
+    assert np.abs(H - Href).max() <= 1e-11 * np.abs(Href).max()
+    assert np.abs(H - op_csr.gram_f64(Vt).cpu().numpy()).max() <= 1e-12 * np.abs(Href).max()
+
+
 def test_vjp_finite_differences(torch_gpu):
     torch = torch_gpu
     X = _counts(300, 200, 0.06, seed=7)

@@ -248,3 +248,57 @@ def test_warm_start_after_weight_change(torch_gpu):
     idx = np.lexsort((-oe, -np.abs(oe)))[:12]
     np.testing.assert_allclose(ev_w, oe[idx], rtol=1e-5, atol=1e-7)
     assert osp.subspace_angle(U_w.cpu().numpy(), ov[:, idx]) < 1e-4
+
+
+@pytest.mark.parametrize("kind", ["small", "float", "large_counts", "mid_counts", "empty_rows"])
+def test_panel_spmm_f64(torch_gpu, kind):
+    """FP64 products over both panel layouts (snapgpu_panel_spmm_f64: several
+    panels each way, every packing, squared values, a row scale or none)
+    against numpy's FP64 sums, to summation rounding."""
+    torch = torch_gpu
+    import ctypes as C
+    from snapatac2_amd import _lib
+    from snapatac2_amd.tl._operator import SpectralOperator, _to_device_csr
+    n, m = 40000, 36000
+    X = small_counts(n, m, 0.004, seed=22)
+    if kind == "float":
+        X = X.astype(np.float32)
+        X.data *= np.float32(0.37)
+    elif kind == "large_counts":
+        X.data[X.indptr[0]:X.indptr[50]] *= 70000
+    elif kind == "mid_counts":
+        X.data[X.indptr[10]:X.indptr[60]] *= 200
+    elif kind == "empty_rows":
+        X = X.tolil()
+        X[100:140, :] = 0
+        X[:, 33000:34000] = 0
+        X = sp.csr_matrix(X, dtype=np.uint32)
+        X.eliminate_zeros()
+    if kind == "small":
+        X.data[:] = np.minimum(X.data, 3)
+    op = SpectralOperator(_to_device_csr(X, torch), use_panels=True)
+    L = _lib.lib()
+    rng = np.random.default_rng(5)
+    Xf = sp.csr_matrix(X, dtype=np.float64)
+    for layout, A, rows, drows in ((op.panel_csr, Xf, n, m), (op.panel_csc, sp.csr_matrix(Xf.T), m, n)):
+        D = rng.standard_normal((drows, 2))
+        s = rng.uniform(0.5, 2.0, rows)
+        ws_b = int(L.snapgpu_panel_spmm_f64_workspace_bytes(layout.n_pairs))
+        ws = torch.empty(ws_b, dtype=torch.uint8, device="cuda")
+        Dd = torch.from_numpy(D).cuda()
+        sd = torch.from_numpy(s).cuda()
+        for square in (0, 1):
+            for scaled in (False, True):
+                out = torch.full((rows, 2), np.nan, dtype=torch.float64, device="cuda")
+                _lib.check(L.snapgpu_panel_spmm_f64(C.byref(layout), Dd.data_ptr(), square,
+                                                    sd.data_ptr() if scaled else None, out.data_ptr(),
+                                                    ws.data_ptr(), ws_b, _lib.stream_ptr()))
+                Aw = A.copy()
+                if square:
+                    Aw.data = Aw.data ** 2
+                ref = Aw @ D
+                if scaled:
+                    ref = s[:, None] * ref
+                got = out.cpu().numpy()
+                assert np.isfinite(got).all()
+                assert np.abs(got - ref).max() <= 1e-13 * max(np.abs(ref).max(), 1e-300), (kind, square, scaled)
