@@ -138,15 +138,25 @@ def spmm_algorithmic_bytes(op, b):
     arrays, whatever the layout reads instead of indptr (tile starts and row
     tables for the block layout, chunk tables and row -> pair lists for the
     panel layout, bucket starts, quad chunk tables and row slots for the window
-    layout, int64 row pointers otherwise).
+    layout, one of each for the mixed layout, int64 row pointers otherwise).
     V read, Z written and read, Y written in b-wide rows; dinv and r per cell."""
     n, m, nnz = op.n, op.m, op.X.nnz
-    if getattr(op, "layout", "panel") == "window":
-        ptrs = 0
-        for L in (op.window_csr, op.window_csc):
-            nbk = L.n_blocks * L.n_windows
-            ptrs += (nbk + 1) * 8 + nbk * 257 * 4 + (L.n_blocks + 1) * 8 + L.n_rows * 4
+
+    def window_ptrs(L):
+        nbk = L.n_blocks * L.n_windows
+        return (nbk + 1) * 8 + nbk * 257 * 4 + (L.n_blocks + 1) * 8 + L.n_rows * 4
+
+    def panel_ptrs(L):
+        return (L.n_chunks + 1) * 16 + (L.n_rows + 1) * 8 + L.n_pairs * 4
+
+    layout = getattr(op, "layout", "panel")
+    if layout == "window":
+        ptrs = window_ptrs(op.window_csr) + window_ptrs(op.window_csc)
         return 2 * nnz * 4 + ptrs + 4 * b * (2 * n + 2 * m) + 8 * n
+    if layout == "mixed":
+        e = 4 if op.panel_csc.packed else 8
+        ptrs = window_ptrs(op.window_csr) + panel_ptrs(op.panel_csc)
+        return nnz * 4 + nnz * e + ptrs + 4 * b * (2 * n + 2 * m) + 8 * n
     if getattr(op, "layout", "panel") == "block":
         ptrs = 0
         e = 0
@@ -159,7 +169,7 @@ def spmm_algorithmic_bytes(op, b):
         e = 0
         for L in (op.panel_csr, op.panel_csc):
             e = max(e, 4 if L.packed else 8)
-            ptrs += (L.n_chunks + 1) * 16 + (L.n_rows + 1) * 8 + L.n_pairs * 4
+            ptrs += panel_ptrs(L)
         return 2 * nnz * e + ptrs + 4 * b * (2 * n + 2 * m) + 8 * n
     return 2 * (nnz * 8 + (n + 1) * 8) + 4 * b * (2 * n + 2 * m) + 8 * n
 
@@ -668,8 +678,8 @@ def operator_only(torch, chunks, cfg, reps, b=16):
     ms = e0.elapsed_time(e1) / reps
     out = {"layout": op.layout, "columns": b, "ms_per_apply": round(ms, 3), "reps": reps, "nnz": int(Xs.nnz),
            "alg_bytes": int(spmm_algorithmic_bytes(op, b))}
-    if op.layout == "panel":
-        out["packed"] = [op.panel_csr.packed, op.panel_csc.packed]
+    if op.layout in ("panel", "mixed"):
+        out["packed"] = [op.panel_csr.packed if op.layout == "panel" else None, op.panel_csc.packed]
         out["gathers_per_s"] = round(2 * Xs.nnz / (ms * 1e-3), 1)
     return out
 
@@ -830,7 +840,7 @@ def main():
                     help="skip the cell x peak and cell x gene matrix measurements")
     ap.add_argument("--import-only", action="store_true",
                     help="run only the fragment-import measurement (its own JSON line)")
-    ap.add_argument("--layout", default=None, choices=["window", "block", "panel"],
+    ap.add_argument("--layout", default=None, choices=["mixed", "window", "block", "panel"],
                     help="operator layout of the large sparse products (default: the package default)")
     ap.add_argument("--parity-cells", type=int, default=100_000,
                     help="cells of the reference-eigsh run in the CPU baseline leg, whose eigenpairs "

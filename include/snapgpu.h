@@ -418,6 +418,15 @@ int snapgpu_spectral_apply_window(const snapgpu_window_layout* csr, const snapgp
                                   const float* d_r32, const float* d_w2, const double* d_dinv,
                                   const double* d_V, int64_t ldv, int32_t b, double* d_Y,
                                   int64_t ldy, void* d_ws, int64_t ws_bytes, void* stream);
+/* Y = A V with the CSC product over the panel layout (rows = features,
+ * dense_rows = cells) and the CSR product over the window layout (rows = cells,
+ * dense_rows = features), b <= 16: the faster layout for each product at a
+ * million cells. The same matvec as snapgpu_spectral_apply (embedding.rs:155-171). */
+int64_t snapgpu_spectral_apply_mixed_workspace_bytes(int64_t n, int64_t m, int64_t pairs_csc);
+int snapgpu_spectral_apply_mixed(const snapgpu_window_layout* csr, const snapgpu_panel_layout* csc,
+                                 const float* d_r32, const float* d_w2, const double* d_dinv,
+                                 const double* d_V, int64_t ldv, int32_t b, double* d_Y,
+                                 int64_t ldy, void* d_ws, int64_t ws_bytes, void* stream);
 
 /* Block-tiled layout (an LDS-staged form of the same products for b <= 16,
  * slower than the panel layout at a million cells). One

@@ -488,6 +488,11 @@ static void launch_product(const snapgpu_panel_layout& L, const float* dense, fl
       L.row_pair_ptr, L.row_pair_ids, L.n_rows, partial, scale, out32, V, ldv, dinv, Y, ldy, b);
 }
 
+void product16_scaled(const snapgpu_panel_layout& L, const float* dense, float* partial, const float* scale,
+                      float* out, hipStream_t st) {
+  launch_product<16, 0>(L, dense, partial, scale, out, nullptr, 0, nullptr, nullptr, 0, 16, st);
+}
+
 // ---------------------------------------------------------------------------
 // FP64 products over the same layout: a dense operand of two FP64 columns (a
 // 16-byte row per stored entry, so one L2 request like the FP32 products') and

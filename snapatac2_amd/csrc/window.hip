@@ -533,4 +533,37 @@ int snapgpu_spectral_apply_window(const snapgpu_window_layout* csr, const snapgp
   return SNAPGPU_OK;
 }
 
+int64_t snapgpu_spectral_apply_mixed_workspace_bytes(int64_t n, int64_t m, int64_t pairs_csc) {
+  return align256(n * 16 * 4) + align256(m * 16 * 4) + align256(std::max<int64_t>(pairs_csc, 1) * 16 * 4) + 256;
+}
+
+int snapgpu_spectral_apply_mixed(const snapgpu_window_layout* csr, const snapgpu_panel_layout* csc,
+                                 const float* d_r32, const float* d_w2, const double* d_dinv, const double* d_V,
+                                 int64_t ldv, int32_t b, double* d_Y, int64_t ldy, void* d_ws, int64_t ws_bytes,
+                                 void* stream) {
+  clear_error();
+  SG_ARG(csr != nullptr && csc != nullptr, "null layout");
+  const int64_t n = csr->n_rows, m = csc->n_rows;
+  SG_ARG(csr->dense_rows == m && csc->dense_rows == n, "layouts do not describe one matrix and its transpose");
+  SG_ARG(csc->packed >= 0 && csc->packed <= 2, "bad packing");
+  SG_ARG(b >= 1 && b <= 16, "block width must be in [1, 16]");
+  SG_ARG(ldv >= b && ldy >= b, "leading dimension < b");
+  SG_ARG(ws_bytes >= snapgpu_spectral_apply_mixed_workspace_bytes(n, m, csc->n_pairs), "workspace too small");
+  hipStream_t st = as_stream(stream);
+  char* q = (char*)(((uintptr_t)d_ws + 255) & ~(uintptr_t)255);
+  float* Vs = (float*)q;
+  q += align256(n * 16 * 4);
+  float* Z = (float*)q;
+  q += align256(m * 16 * 4);
+  float* part = (float*)q;
+  scale_in_kernel<16><<<(unsigned)ceil_div(n * 16, 256), 256, 0, st>>>(d_V, ldv, n, b, d_r32, Vs);
+  // Z = w^2 o (X^T Vs) over the CSC panels, then Y = r o (X Z) - dinv o V over the CSR windows
+  panel::product16_scaled(*csc, Vs, part, d_w2, Z, st);
+  product_kernel<1><<<(unsigned)csr->n_blocks, kThreads, 0, st>>>(
+      csr->ent, csr->bucket_start, csr->quad_off, csr->n_windows, csr->block_row, csr->row_slot, Z, d_r32,
+      nullptr, d_V, ldv, d_dinv, d_Y, ldy, b);
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
 }  // extern "C"

@@ -15,7 +15,7 @@ from .._device import DeviceCSR
 
 TRANSPOSE_CHUNK = 1 << 28  # entries sorted per transpose chunk
 BLOCK_TARGET = 512  # blocks per direction of the block-tiled layout (about two per CU)
-DEFAULT_TILED = "panel"  # layout of large operators
+DEFAULT_TILED = "mixed"  # layout of large operators (CSC panels + CSR windows; panels where windows cannot hold the counts)
 
 
 def _to_device_csr(X, torch) -> DeviceCSR:
@@ -80,11 +80,14 @@ class SpectralOperator:
 
     def __init__(self, X: DeviceCSR, feature_weights=None, use_panels: bool | None = None,
                  layout: str | None = None):
-        """layout: 'block' (LDS-tiled products, b <= 16 per launch), 'panel'
-        (L2-sized panels, b <= 32) or 'csr' (row-parallel products); by default
-        DEFAULT_TILED when either dense operand spans more than one 32,768-row
-        panel, else 'csr'. use_panels=True / False stands for the default tiled layout /
-        'csr'."""
+        """layout: 'mixed' (the CSC product over L2-sized panels, the CSR
+        product over 1024-row windows, b <= 16 per launch), 'panel' (both over
+        panels, b <= 32), 'window' (both over windows, b <= 16), 'block'
+        (LDS-tiled products, b <= 16) or 'csr' (row-parallel products); by
+        default DEFAULT_TILED when either dense operand spans more than one
+        32,768-row panel, else 'csr'. use_panels=True / False stands for the
+        default tiled layout / 'csr'. The window layouts need integer counts
+        below 256; other matrices get 'panel' instead."""
         torch = _lib.require_gpu()
         L = _lib.lib()
         self.torch = torch
@@ -130,23 +133,28 @@ class SpectralOperator:
         if layout is None:
             tiled = (pc > 1 or pr > 1) if use_panels is None else bool(use_panels)
             layout = DEFAULT_TILED if tiled else "csr"
-        if layout not in ("window", "block", "panel", "csr"):
-            raise ValueError("layout must be 'window', 'block', 'panel' or 'csr'")
-        if layout == "window" and self.value_type != 0:
+        if layout not in ("mixed", "window", "block", "panel", "csr"):
+            raise ValueError("layout must be 'mixed', 'window', 'block', 'panel' or 'csr'")
+        if layout in ("mixed", "window") and self.value_type != 0:
             layout = "panel"  # the window entries hold integer counts
         self.layout = layout
         self.use_panels = layout != "csr"
         self._panel_keep = []
-        if layout == "window":
+        if layout in ("mixed", "window"):
             try:
                 self.window_csr = self._build_window(X.indptr, X.indices, X.data, n, m, X.nnz)
-                self.window_csc = self._build_window(self.colptr, self.rowidx, self.cval, m, n, X.nnz)
+                if layout == "window":
+                    self.window_csc = self._build_window(self.colptr, self.rowidx, self.cval, m, n, X.nnz)
             except _lib.CapacityError:
                 # counts >= 256 or too many windows: the panel layout takes them
                 self._panel_keep = []
                 layout = self.layout = "panel"
             if layout == "window":
                 self._pws_b = int(L.snapgpu_spectral_apply_window_workspace_bytes(n, m))
+                self._pws = torch.empty(self._pws_b, dtype=torch.uint8, device=dev)
+            elif layout == "mixed":
+                self.panel_csc = self._build_panel(self.colptr, self.rowidx, self.cval, m, n, X.nnz)
+                self._pws_b = int(L.snapgpu_spectral_apply_mixed_workspace_bytes(n, m, self.panel_csc.n_pairs))
                 self._pws = torch.empty(self._pws_b, dtype=torch.uint8, device=dev)
         if layout == "panel":
             self.panel_csr = self._build_panel(X.indptr, X.indices, X.data, n, m, X.nnz)
@@ -326,7 +334,7 @@ class SpectralOperator:
         L = _lib.lib()
         X = self.X
         b = V.shape[1] if b is None else b
-        wmax = 16 if self.layout in ("block", "window") else 32
+        wmax = 16 if self.layout in ("block", "window", "mixed") else 32
         if b > wmax:
             for c0 in range(0, b, wmax):
                 self.apply(V[:, c0:c0 + wmax], Y[:, c0:c0 + wmax])
@@ -335,7 +343,12 @@ class SpectralOperator:
         if self.events is not None:
             ev = (self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True))
             ev[0].record()
-        if self.layout == "window":
+        if self.layout == "mixed":
+            _lib.check(L.snapgpu_spectral_apply_mixed(
+                C.byref(self.window_csr), C.byref(self.panel_csc), self.r32.data_ptr(),
+                self.w2.data_ptr(), self.dinv.data_ptr(), V.data_ptr(), V.stride(0), b, Y.data_ptr(),
+                Y.stride(0), self._pws.data_ptr(), self._pws_b, _lib.stream_ptr()))
+        elif self.layout == "window":
             _lib.check(L.snapgpu_spectral_apply_window(
                 C.byref(self.window_csr), C.byref(self.window_csc), self.r32.data_ptr(),
                 self.w2.data_ptr(), self.dinv.data_ptr(), V.data_ptr(), V.stride(0), b, Y.data_ptr(),
@@ -416,7 +429,7 @@ class SpectralOperator:
         b = U.shape[1]
         rU = (self.r64[:, None] * U).contiguous()
         P = torch.empty((self.m, b), dtype=torch.float64, device=U.device)
-        if self.layout == "panel":
+        if self.layout in ("panel", "mixed"):
             # two columns at a time over the CSC panel layout (L2-sized slices of rU)
             ws_b = int(L.snapgpu_panel_spmm_f64_workspace_bytes(self.panel_csc.n_pairs))
             if self._pws_b >= ws_b:

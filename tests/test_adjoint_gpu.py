@@ -87,7 +87,7 @@ def test_gram_f64_panel_layout(torch_gpu, b):
     X = _counts(40000, 36000, 0.004, seed=31)
     w = _weights(X, 31)
     Xd = _to_device_csr(X, torch)
-    op = SpectralOperator(Xd, w, use_panels=True)
+    op = SpectralOperator(Xd, w, layout="panel")
     op_csr = SpectralOperator(Xd, w, use_panels=False)
     assert op.layout == "panel" and op_csr.layout == "csr"
     V = np.random.default_rng(9).standard_normal((40000, b))

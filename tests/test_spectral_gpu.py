@@ -142,7 +142,7 @@ def test_panel_segmented_operator(torch_gpu):
     from snapatac2_amd.tl._operator import SpectralOperator, _to_device_csr
     X = small_counts(70000, 80000, 0.0015, seed=12)
     Xd = _to_device_csr(X, torch)
-    op_p = SpectralOperator(Xd, use_panels=True)
+    op_p = SpectralOperator(Xd, layout="panel")
     op_1 = SpectralOperator(Xd, use_panels=False)
     Xh, dinv, _, _ = osp.operator_parts(X)
     rng = np.random.default_rng(1)
@@ -171,9 +171,9 @@ def test_panel_fast_product_bitwise(torch_gpu, monkeypatch):
     X = small_counts(70000, 80000, 0.0015, seed=31)
     X.data[::97] = 127  # the largest packed-2 count
     Xd = _to_device_csr(X, torch)
-    op2 = SpectralOperator(Xd, use_panels=True)
+    op2 = SpectralOperator(Xd, layout="panel")
     monkeypatch.setattr(SpectralOperator, "panel_pack_max", 1)
-    op1 = SpectralOperator(Xd, use_panels=True)
+    op1 = SpectralOperator(Xd, layout="panel")
     assert op2.panel_csr.packed == 2 and op2.panel_csc.packed == 2
     assert op1.panel_csr.packed == 1 and op1.panel_csc.packed == 1
     rng = np.random.default_rng(4)
@@ -214,7 +214,7 @@ def test_panel_layout_variants(torch_gpu, kind):
         X = sp.csr_matrix(X, dtype=np.uint32)
         X.eliminate_zeros()
     Xd = _to_device_csr(X, torch)
-    op = SpectralOperator(Xd, use_panels=True)
+    op = SpectralOperator(Xd, layout="panel")
     ref_op = SpectralOperator(Xd, use_panels=False)
     packed = {"small": 2, "empty_rows": 2, "mid_counts": 1, "float": 0, "large_counts": 0}[kind]
     assert op.panel_csr.packed == packed and op.panel_csc.packed == packed
@@ -276,7 +276,7 @@ def test_panel_spmm_f64(torch_gpu, kind):
         X.eliminate_zeros()
     if kind == "small":
         X.data[:] = np.minimum(X.data, 3)
-    op = SpectralOperator(_to_device_csr(X, torch), use_panels=True)
+    op = SpectralOperator(_to_device_csr(X, torch), layout="panel")
     L = _lib.lib()
     rng = np.random.default_rng(5)
     Xf = sp.csr_matrix(X, dtype=np.float64)

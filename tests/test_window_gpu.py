@@ -169,3 +169,32 @@ def test_window_eigenpairs(torch_gpu):
     ev, _, _ = eigsh_lm(op, 10, seed=0)
     ev_p, _, _ = eigsh_lm(op_p, 10, seed=0)
     assert np.allclose(np.sort(ev), np.sort(ev_p), rtol=1e-7, atol=0)
+
+
+@pytest.mark.parametrize("b", [16, 3, 21])
+def test_mixed_layout(torch_gpu, b):
+    """The default layout of large operators: CSC product over panels, CSR
+    product over windows. Against the FP64 oracle and the panel layout, bitwise
+    reproducible; matrices the windows cannot hold take the panel layout."""
+    torch = torch_gpu
+    from snapatac2_amd.tl._operator import SpectralOperator, _to_device_csr
+    X = _counts(70000, 80000, 0.0015, seed=13, heavy_rows=(5, 69999), heavy_cols=(0, 79999))
+    Xd = _to_device_csr(X, torch)
+    op = SpectralOperator(Xd)
+    assert op.layout == "mixed" and op.window_csr.n_blocks > 1 and op.panel_csc.n_chunks > 1
+    op_p = SpectralOperator(Xd, layout="panel")
+    Xh, dinv, _, _ = osp.operator_parts(X)
+    V = np.random.default_rng(b).standard_normal((70000, b))
+    Vt = torch.from_numpy(V).cuda()
+    Y = op.matmat(Vt)
+    ref = Xh @ (Xh.T @ V) - dinv[:, None] * V
+    scale = np.abs(ref).max()
+    assert np.abs(Y.cpu().numpy() - ref).max() / scale < 1e-5
+    assert np.abs(Y.cpu().numpy() - op_p.matmat(Vt).cpu().numpy()).max() / scale < 1e-5
+    assert torch.equal(Y, op.matmat(Vt))
+    if b == 3:
+        Xc = X.copy()
+        Xc.data[::50] = 300  # a count the window entry cannot hold
+        assert SpectralOperator(_to_device_csr(Xc, torch), layout="mixed").layout == "panel"
+        Xf = sp.csr_matrix(X, dtype=np.float32) * np.float32(0.5)
+        assert SpectralOperator(_to_device_csr(Xf, torch), layout="mixed").layout == "panel"
