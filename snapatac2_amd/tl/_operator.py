@@ -16,6 +16,9 @@ from .._device import DeviceCSR
 TRANSPOSE_CHUNK = 1 << 28  # entries sorted per transpose chunk
 BLOCK_TARGET = 512  # blocks per direction of the block-tiled layout (about two per CU)
 DEFAULT_TILED = "mixed"  # layout of large operators (CSC panels + CSR windows; panels where windows cannot hold the counts)
+# the CSR windows pay off when they fill the chip: with fewer workgroups (about two per CU
+# of an MI355X) the default takes the panel layout instead
+MIXED_MIN_BLOCKS = 512
 
 
 def _to_device_csr(X, torch) -> DeviceCSR:
@@ -130,6 +133,7 @@ class SpectralOperator:
         # panel layouts (dense operands larger than one L2-sized panel)
         pc = int(L.snapgpu_panel_count(m))
         pr = int(L.snapgpu_panel_count(n))
+        chosen = layout is None
         if layout is None:
             tiled = (pc > 1 or pr > 1) if use_panels is None else bool(use_panels)
             layout = DEFAULT_TILED if tiled else "csr"
@@ -141,13 +145,18 @@ class SpectralOperator:
         self.use_panels = layout != "csr"
         self._panel_keep = []
         if layout in ("mixed", "window"):
+            fits = True
             try:
                 self.window_csr = self._build_window(X.indptr, X.indices, X.data, n, m, X.nnz)
                 if layout == "window":
                     self.window_csc = self._build_window(self.colptr, self.rowidx, self.cval, m, n, X.nnz)
             except _lib.CapacityError:
-                # counts >= 256 or too many windows: the panel layout takes them
+                fits = False  # counts >= 256 or too many windows
+            if fits and layout == "mixed" and chosen and self.window_csr.n_blocks < MIXED_MIN_BLOCKS:
+                fits = False  # the default: too few window workgroups to fill the device
+            if not fits:  # the panel layout takes them
                 self._panel_keep = []
+                self.window_csr = self.window_csc = None
                 layout = self.layout = "panel"
             if layout == "window":
                 self._pws_b = int(L.snapgpu_spectral_apply_window_workspace_bytes(n, m))

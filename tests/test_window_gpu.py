@@ -172,14 +172,21 @@ def test_window_eigenpairs(torch_gpu):
 
 
 @pytest.mark.parametrize("b", [16, 3, 21])
-def test_mixed_layout(torch_gpu, b):
+def test_mixed_layout(torch_gpu, monkeypatch, b):
     """The default layout of large operators: CSC product over panels, CSR
     product over windows. Against the FP64 oracle and the panel layout, bitwise
-    reproducible; matrices the windows cannot hold take the panel layout."""
+    reproducible; matrices the windows cannot hold take the panel layout, and so
+    does the default when the CSR windows come to fewer workgroups than
+    MIXED_MIN_BLOCKS (this matrix: a few dozen)."""
     torch = torch_gpu
+    from snapatac2_amd.tl import _operator
     from snapatac2_amd.tl._operator import SpectralOperator, _to_device_csr
     X = _counts(70000, 80000, 0.0015, seed=13, heavy_rows=(5, 69999), heavy_cols=(0, 79999))
     Xd = _to_device_csr(X, torch)
+    if b == 16:
+        assert SpectralOperator(Xd).layout == "panel"
+        assert SpectralOperator(Xd, layout="mixed").layout == "mixed"
+    monkeypatch.setattr(_operator, "MIXED_MIN_BLOCKS", 1)
     op = SpectralOperator(Xd)
     assert op.layout == "mixed" and op.window_csr.n_blocks > 1 and op.panel_csc.n_chunks > 1
     op_p = SpectralOperator(Xd, layout="panel")
