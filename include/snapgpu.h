@@ -427,6 +427,21 @@ int snapgpu_spectral_apply_mixed(const snapgpu_window_layout* csr, const snapgpu
                                  const float* d_r32, const float* d_w2, const double* d_dinv,
                                  const double* d_V, int64_t ldv, int32_t b, double* d_Y,
                                  int64_t ldy, void* d_ws, int64_t ws_bytes, void* stream);
+/* snapgpu_spectral_vjp_w with its two FP32 sparse products over the mixed layout (the CSC
+ * panel layout and the CSR window layout of snapgpu_spectral_apply_mixed), for
+ * integer counts and b <= 8; the FP64 passes read the CSR and CSC arrays as
+ * above. d_pws: snapgpu_spectral_vjp_mixed_workspace_bytes(csc n_pairs) bytes for
+ * the panel product's pair sums (the mixed apply's workspace is large enough). */
+int64_t snapgpu_spectral_vjp_mixed_workspace_bytes(int64_t pairs_csc);
+int snapgpu_spectral_vjp_w_mixed(const snapgpu_window_layout* csr_windows,
+                                 const snapgpu_panel_layout* csc_panels, const int64_t* d_indptr,
+                                 const int32_t* d_idx, const uint32_t* d_cnt, int64_t n,
+                                 const int64_t* d_colptr, const int32_t* d_rowidx,
+                                 const uint32_t* d_ccnt, int64_t m, const double* d_w,
+                                 const double* d_s, const double* d_dinv, const float* d_r32,
+                                 const float* d_w2, const double* d_V, int64_t ldv, const double* d_G,
+                                 int64_t ldg, int32_t b, double* d_dw, void* d_ws, int64_t ws_bytes,
+                                 void* d_pws, int64_t pws_bytes, void* stream);
 
 /* Block-tiled layout (an LDS-staged form of the same products for b <= 16,
  * slower than the panel layout at a million cells). One

@@ -84,6 +84,9 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_spectral_apply_window": (C.c_int, [_p, _p, _p, _p, _p, _p, _i64, _i32, _p, _i64, _p, _i64, _p]),
     "snapgpu_spectral_apply_mixed_workspace_bytes": (_i64, [_i64, _i64, _i64]),
     "snapgpu_spectral_apply_mixed": (C.c_int, [_p, _p, _p, _p, _p, _p, _i64, _i32, _p, _i64, _p, _i64, _p]),
+    "snapgpu_spectral_vjp_mixed_workspace_bytes": (_i64, [_i64]),
+    "snapgpu_spectral_vjp_w_mixed": (C.c_int, [_p, _p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _p, _p,
+                                               _p, _i64, _p, _i64, _i32, _p, _p, _i64, _p, _i64, _p]),
     "snapgpu_spectral_apply_block_workspace_bytes": (_i64, [_i64, _i64]),
     "snapgpu_spectral_apply_block": (C.c_int, [_p, _p, _p, _p, _p, _p, _i64, _i32, _p, _i64, _p, _i64, _p]),
     "snapgpu_gemm_tn_workspace_bytes": (_i64, [_i64, _i32, _i32]),

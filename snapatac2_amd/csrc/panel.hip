@@ -422,7 +422,7 @@ __global__ void __launch_bounds__(256) product16_kernel(const uint32_t* __restri
 }
 
 // out row = sum of the row's pairs in panel order, then the epilogue:
-//   EPI 0: out32[row][c] = scale[row] * sum                     (FP32, stride BW)
+//   EPI 0: out32[row][c] = scale[row] * sum (sum when scale is null) (FP32, stride BW)
 //   EPI 1: Y[row][c] = scale[row] * sum - dinv[row] * V[row][c]  (FP64, c < b)
 template <int BW, int EPI>
 __global__ void __launch_bounds__(256) reduce_kernel(const int64_t* __restrict__ rp_ptr,
@@ -451,7 +451,7 @@ __global__ void __launch_bounds__(256) reduce_kernel(const int64_t* __restrict__
   }
   for (; i < i1; ++i) acc += partial[(int64_t)rp_ids[i] * BW + c];
   if (EPI == 0) {
-    out32[row * BW + c] = acc * scale[row];
+    out32[row * BW + c] = scale ? acc * scale[row] : acc;
   } else if (c < b) {
     Y[row * ldy + c] = (double)(acc * scale[row]) - dinv[row] * V[row * ldv + c];
   }

@@ -924,12 +924,20 @@ static int64_t vjp_ws_bytes(int64_t n, int64_t m) {
   return 2 * al(n * kB * 4) + 2 * al(m * kB * 4) + 5 * al(n * 8) + 4 * al(m * 8) + 256;
 }
 
+// The two FP32 products of the adjoint over the mixed layout (CSC panels, CSR
+// windows) instead of the row-parallel spmm_kernel; 16-column rows only.
+struct MixedProducts {
+  const snapgpu_window_layout* csr;
+  const snapgpu_panel_layout* csc;
+  float* partial;  // the CSC panel product's pair sums
+};
+
 template <int BW, bool FV>
 static void launch_vjp(const int64_t* indptr, const int32_t* idx, const uint32_t* cnt, int64_t n,
                        const int64_t* colptr, const int32_t* rowidx, const uint32_t* ccnt, int64_t m,
                        const double* u, const double* dinv, const double* w, const float* r32,
                        const float* w2, const double* V, int64_t ldv, const double* G, int64_t ldg,
-                       int b, double* dw, void* ws, hipStream_t st) {
+                       int b, double* dw, void* ws, hipStream_t st, const MixedProducts* mixed = nullptr) {
   char* q = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
   auto take = [&q](int64_t bytes) {
     char* p = q;
@@ -954,13 +962,19 @@ static void launch_vjp(const int64_t* indptr, const int32_t* idx, const uint32_t
     // V and G side by side: one CSC and one CSR pass
     scale_in_pair_kernel<BW><<<(unsigned)ceil_div(n * BW, 256), 256, 0, st>>>(V, ldv, G, ldg, n, b, r32, Vs);
     // [P | Q] = X^T (r o [V | G])
-    spmm_kernel<BW, 0, FV><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, Vs, nullptr, Pm, nullptr, 0,
-                                               nullptr, nullptr, 0, 2 * b);
+    if (BW == 16 && mixed)
+      panel::product16_scaled(*mixed->csc, Vs, mixed->partial, nullptr, Pm, st);
+    else
+      spmm_kernel<BW, 0, FV><<<gm, 256, 0, st>>>(colptr, rowidx, ccnt, m, Vs, nullptr, Pm, nullptr, 0,
+                                                 nullptr, nullptr, 0, 2 * b);
     rowdot_f32_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(Pm, Pm + b, m, BW, b, h);
     scale_rows_f32_kernel<<<(unsigned)ceil_div(m * BW, 256), 256, 0, st>>>(Pm, m, BW, w2);
     // X (w^2 o [P | Q]) into Vs (free after the CSC pass)
-    spmm_kernel<BW, 0, FV><<<gn, 256, 0, st>>>(indptr, idx, cnt, n, Pm, nullptr, Vs, nullptr, 0,
-                                               nullptr, nullptr, 0, 2 * b);
+    if (BW == 16 && mixed)
+      window::product16_scaled(*mixed->csr, Pm, nullptr, Vs, st);
+    else
+      spmm_kernel<BW, 0, FV><<<gn, 256, 0, st>>>(indptr, idx, cnt, n, Pm, nullptr, Vs, nullptr, 0,
+                                                 nullptr, nullptr, 0, 2 * b);
     adj_rows_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(G, ldg, V, ldv, Vs, Vs + b, BW, b, n, u,
                                                                 dinv, alpha, gamma, t);
   } else {
@@ -1332,6 +1346,39 @@ int snapgpu_spectral_vjp_w(const int64_t* d_indptr, const int32_t* d_idx, const 
     SG_VJP(32)
   }
 #undef SG_VJP
+  SG_LAUNCH_CHECK();
+  return SNAPGPU_OK;
+}
+
+int64_t snapgpu_spectral_vjp_mixed_workspace_bytes(int64_t pairs_csc) {
+  return (std::max<int64_t>(pairs_csc, 1) * 16 * 4 + 255) / 256 * 256 + 256;
+}
+
+int snapgpu_spectral_vjp_w_mixed(const snapgpu_window_layout* csr_windows, const snapgpu_panel_layout* csc_panels,
+                                 const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_cnt, int64_t n,
+                                 const int64_t* d_colptr, const int32_t* d_rowidx, const uint32_t* d_ccnt,
+                                 int64_t m, const double* d_w, const double* d_s, const double* d_dinv,
+                                 const float* d_r32, const float* d_w2, const double* d_V, int64_t ldv,
+                                 const double* d_G, int64_t ldg, int32_t b, double* d_dw, void* d_ws,
+                                 int64_t ws_bytes, void* d_pws, int64_t pws_bytes, void* stream) {
+  clear_error();
+  SG_ARG(csr_windows != nullptr && csc_panels != nullptr, "null layout");
+  SG_ARG(n >= 1 && m >= 1, "bad shape");
+  SG_ARG(csr_windows->n_rows == n && csr_windows->dense_rows == m && csc_panels->n_rows == m &&
+             csc_panels->dense_rows == n,
+         "layouts do not describe this matrix and its transpose");
+  SG_ARG(csc_panels->packed >= 0 && csc_panels->packed <= 2 && csc_panels->value_type == 0,
+         "the mixed layout holds integer counts");
+  SG_ARG(b >= 1 && b <= 8, "block width must be in [1, 8] (V and G side by side in 16 columns)");
+  SG_ARG(ldv >= b && ldg >= b, "leading dimension < b");
+  SG_ARG(d_ws != nullptr && ws_bytes >= vjp_ws_bytes(n, m), "workspace too small");
+  SG_ARG(d_pws != nullptr && pws_bytes >= snapgpu_spectral_vjp_mixed_workspace_bytes(csc_panels->n_pairs),
+         "panel workspace too small");
+  hipStream_t st = as_stream(stream);
+  MixedProducts mp{csr_windows, csc_panels,
+                   reinterpret_cast<float*>(((uintptr_t)d_pws + 255) & ~(uintptr_t)255)};
+  launch_vjp<16, false>(d_indptr, d_idx, d_cnt, n, d_colptr, d_rowidx, d_ccnt, m, d_s, d_dinv, d_w, d_r32, d_w2,
+                        d_V, ldv, d_G, ldg, b, d_dw, d_ws, st, &mp);
   SG_LAUNCH_CHECK();
   return SNAPGPU_OK;
 }

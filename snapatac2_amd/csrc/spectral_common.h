@@ -66,9 +66,17 @@ __global__ void scale_in_kernel(const double* V, int64_t ldv, int64_t n, int b, 
 
 namespace panel {
 // out[row][c] = scale[row] * (sum over the layout's entries of value * dense row),
-// 16-column FP32 rows, through the panel product and its per-row reduction
-// (panel.hip): the CSC product of the mixed layout (window.hip).
+// 16-column FP32 rows (scale null: 1), through the panel product and its per-row
+// reduction (panel.hip): the CSC product of the mixed layout (window.hip, and the
+// adjoint over that layout in spectral.hip).
 void product16_scaled(const snapgpu_panel_layout& L, const float* dense, float* partial, const float* scale,
                       float* out, hipStream_t st);
 }  // namespace panel
+
+namespace window {
+// The same over the window layout (window.hip): the CSR product of the adjoint over
+// the mixed layout.
+void product16_scaled(const snapgpu_window_layout& L, const float* dense, const float* scale, float* out,
+                      hipStream_t st);
+}  // namespace window
 }  // namespace snapgpu

@@ -225,7 +225,7 @@ __global__ void quad_split_kernel(const int64_t* __restrict__ bstart, const uint
 // The product. Workgroup b walks the windows; quad k sums its chunk's pieces
 // (4 lanes, lane j holding columns 4j..4j+3) and adds each piece to its slot;
 // a barrier closes each window. The epilogue adds each row's slots in order:
-//   EPI 0: out32[row][c] = scale[row] * sum                     (FP32, stride 16)
+//   EPI 0: out32[row][c] = scale[row] * sum (sum when scale is null) (FP32, stride 16)
 //   EPI 1: Y[row][c] = scale[row] * sum - dinv[row] * V[row][c]  (FP64, c < b)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void add_piece(float4* acc, uint32_t slot, int q, const float4& a) {
@@ -351,7 +351,7 @@ __global__ void __launch_bounds__(kThreads) product_kernel(
       t.z += a.z;
       t.w += a.w;
     }
-    const float sc = scale[row];
+    const float sc = scale ? scale[row] : 1.f;
     const float o[4] = {t.x * sc, t.y * sc, t.z * sc, t.w * sc};
     if (EPI == 0) {
       *reinterpret_cast<float4*>(out32 + row * 16 + q * 4) = make_float4(o[0], o[1], o[2], o[3]);
@@ -366,6 +366,13 @@ __global__ void __launch_bounds__(kThreads) product_kernel(
 }
 
 static int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
+void product16_scaled(const snapgpu_window_layout& L, const float* dense, const float* scale, float* out,
+                      hipStream_t st) {
+  product_kernel<0><<<(unsigned)L.n_blocks, kThreads, 0, st>>>(L.ent, L.bucket_start, L.quad_off, L.n_windows,
+                                                               L.block_row, L.row_slot, dense, scale, out, nullptr,
+                                                               0, nullptr, nullptr, 0, 16);
+}
 
 struct PlanWs {
   int32_t* s;          // nrows

@@ -323,10 +323,24 @@ class SpectralOperator:
             self._vjp_ws_b = int(L.snapgpu_spectral_vjp_workspace_bytes(self.n, self.m))
             self._vjp_ws = torch.empty(self._vjp_ws_b, dtype=torch.uint8, device=self.w.device)
         dw = torch.zeros(self.m, dtype=torch.float64, device=self.w.device)
-        part = torch.empty_like(dw) if V.shape[1] > 32 else dw
-        for c0 in range(0, V.shape[1], 32):
-            b = min(32, V.shape[1] - c0)
+        mixed = self.layout == "mixed"
+        step = 8 if mixed else 32  # the mixed products take V and G side by side in 16 columns
+        part = torch.empty_like(dw) if V.shape[1] > step else dw
+        for c0 in range(0, V.shape[1], step):
+            b = min(step, V.shape[1] - c0)
             Vc, Gc = V[:, c0:c0 + b], G[:, c0:c0 + b]
+            if mixed:
+                _lib.check(L.snapgpu_spectral_vjp_w_mixed(
+                    C.byref(self.window_csr), C.byref(self.panel_csc),
+                    X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), self.n,
+                    self.colptr.data_ptr(), self.rowidx.data_ptr(), self.cval.data_ptr(), self.m,
+                    self.w.data_ptr(), self.s.data_ptr(), self.dinv.data_ptr(), self.r32.data_ptr(),
+                    self.w2.data_ptr(), Vc.data_ptr(), Vc.stride(0), Gc.data_ptr(), Gc.stride(0), b,
+                    part.data_ptr(), self._vjp_ws.data_ptr(), self._vjp_ws_b, self._pws.data_ptr(),
+                    self._pws_b, _lib.stream_ptr()))
+                if part is not dw:
+                    dw += part
+                continue
             _lib.check(L.snapgpu_spectral_vjp_w(
                 X.indptr.data_ptr(), X.indices.data_ptr(), X.data.data_ptr(), self.n,
                 self.colptr.data_ptr(), self.rowidx.data_ptr(), self.cval.data_ptr(), self.m,

@@ -98,6 +98,29 @@ def test_gram_f64_panel_layout(torch_gpu, b):
     assert np.abs(H - op_csr.gram_f64(Vt).cpu().numpy()).max() <= 1e-12 * np.abs(Href).max()
 
 
+@pytest.mark.parametrize("b", [2, 5, 11])
+def test_vjp_mixed_layout(torch_gpu, b):
+    """The adjoint with its FP32 products over the mixed layout (CSC panels, CSR
+    windows; V and G side by side in 16 columns, 8 columns per call) against
+    the closed form and the row-parallel adjoint."""
+    torch = torch_gpu
+    from snapatac2_amd.tl._operator import SpectralOperator, _to_device_csr
+    X = _counts(40000, 36000, 0.004, seed=41)
+    w = _weights(X, 41)
+    Xd = _to_device_csr(X, torch)
+    op = SpectralOperator(Xd, w, layout="mixed")
+    op_csr = SpectralOperator(Xd, w, layout="csr")
+    assert op.layout == "mixed"
+    rng = np.random.default_rng(400 + b)
+    V = rng.standard_normal((40000, b))
+    G = rng.standard_normal((40000, b))
+    Vt, Gt = torch.from_numpy(V).cuda(), torch.from_numpy(G).cuda()
+    dw = op.vjp_w(Vt, Gt).cpu().numpy()
+    assert _rel(dw, adj.closed_form_grad(X, w, V, G)) < 1e-5
+    assert _rel(dw, op_csr.vjp_w(Vt, Gt).cpu().numpy()) < 1e-5
+    assert np.array_equal(dw, op.vjp_w(Vt, Gt).cpu().numpy())  # fixed summation order
+
+
 def test_vjp_finite_differences(torch_gpu):
     torch = torch_gpu
     X = _counts(300, 200, 0.06, seed=7)
