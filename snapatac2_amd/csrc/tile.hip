@@ -165,25 +165,15 @@ __device__ __forceinline__ void bitonic128(uint32_t (&k)[2], int lane) {
   }
 }
 
-// Number of entries < key (strict=true) or <= key (strict=false) in a sorted LDS array.
-__device__ __forceinline__ int lds_rank(const uint32_t* arr, int n, uint32_t key, bool strict) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    uint32_t v = arr[mid];
-    bool go_right = strict ? (v < key) : (v <= key);
-    if (go_right) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
 // Number of entries below key (STRICT) or at or below key in a sorted LDS
-// array of n <= 64 entries: a fixed seven-step search, no divergent loop.
-template <bool STRICT>
-__device__ __forceinline__ int lds_count64(const uint32_t* arr, int n, uint32_t key) {
+// array of n <= CAP entries (CAP a power of two): a fixed search of
+// log2(CAP) + 1 steps, the same for every lane, so no divergent loop.
+template <bool STRICT, int CAP>
+__device__ __forceinline__ int lds_count(const uint32_t* arr, int n, uint32_t key) {
+  static_assert((CAP & (CAP - 1)) == 0, "CAP must be a power of two");
   int lo = 0;
 #pragma unroll
-  for (int step = 64; step > 0; step >>= 1) {
+  for (int step = CAP; step > 0; step >>= 1) {
     if (lo + step <= n) {
       const uint32_t v = arr[lo + step - 1];
       if (STRICT ? v < key : v <= key) lo += step;
@@ -222,8 +212,8 @@ __device__ __forceinline__ bool presort_pairs(uint32_t (&k)[2], int lane, WaveLd
     if (v0) L->nbin[i0] = k[0];
     if (v1) L->nbin[64 + i1] = k[1];
     wave_sync();
-    const int r0 = v0 ? i0 + lds_count64<true>(L->nbin + 64, n1, k[0]) : 0;
-    const int r1 = v1 ? i1 + lds_count64<false>(L->nbin, n0, k[1]) : 0;
+    const int r0 = v0 ? i0 + lds_count<true, 64>(L->nbin + 64, n1, k[0]) : 0;
+    const int r1 = v1 ? i1 + lds_count<false, 64>(L->nbin, n0, k[1]) : 0;
     L->mbin[lane] = kInf;
     L->mbin[64 + lane] = kInf;
     wave_sync();
@@ -311,18 +301,18 @@ __device__ void merge_round(CellState<CAP, FILL, IdxT>& s, const Params& p, uint
   wave_sync();
   const int np = s.np;
   if (st0) {
-    int r = idx0 + lds_rank(L->pbin, np, k[0], false);
+    int r = idx0 + lds_count<false, CAP>(L->pbin, np, k[0]);
     L->mbin[r] = k[0];
     if (FILL) L->mcnt[r] = cnt0;
   }
   if (st1) {
-    int r = idx1 + lds_rank(L->pbin, np, k[1], false);
+    int r = idx1 + lds_count<false, CAP>(L->pbin, np, k[1]);
     L->mbin[r] = k[1];
     if (FILL) L->mcnt[r] = cnt1;
   }
   for (int j = lane; j < np; j += 64) {
     uint32_t b = L->pbin[j];
-    int r = j + lds_rank(L->nbin, nN, b, true);
+    int r = j + lds_count<true, 128>(L->nbin, nN, b);
     L->mbin[r] = b;
     if (FILL) L->mcnt[r] = L->pcnt[j];
   }
