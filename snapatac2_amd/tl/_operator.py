@@ -12,6 +12,7 @@ import numpy as np
 
 from .. import _lib
 from .._device import DeviceCSR
+from ._lanczos import _Linalg
 
 TRANSPOSE_CHUNK = 1 << 28  # entries sorted per transpose chunk
 BLOCK_TARGET = 512  # blocks per direction of the block-tiled layout (about two per CU)
@@ -445,7 +446,6 @@ class SpectralOperator:
         float-valued matrices."""
         if self.value_type != 0:
             return None
-        from ._lanczos import _Linalg
         torch = self.torch
         L = _lib.lib()
         U = U.to(torch.float64).contiguous()
