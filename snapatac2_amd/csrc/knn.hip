@@ -325,6 +325,48 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
 // as for a point that was seen and not kept. Processed chunks are shifted by
 // the centre, given their augmented coordinate |x - c|^2, split into three
 // bf16 terms and staged in LDS; every wave then runs its 16 x 16 MFMA tiles.
+// Accepted candidates of the centred kernel wait in a per-lane queue of kQueue
+// entries and enter the sorted list together. list_insert runs for the whole
+// wave whenever one lane inserts, so inserting from the queues costs one pass
+// per queued entry of the fullest lane instead of one pass per (tile, value) at
+// which any of the 64 lanes accepted. While entries wait, the acceptance
+// threshold is the list's older (lower) minimum, which only lets a few more
+// candidates in; list_insert keeps the best kList whatever the order, and a
+// point turned away was below a minimum the final list exceeds.
+constexpr int kQueue = 4;
+
+struct CandQueue {
+  float v[kQueue];
+  int32_t p[kQueue];
+  int n;
+};
+
+__device__ __forceinline__ void queue_flush(CandQueue& q, float (&L)[kList], int32_t (&I)[kList]) {
+#pragma unroll
+  for (int t = 0; t < kQueue; ++t) {
+    const bool has = t < q.n;
+    if (__ballot(has)) {
+      if (has) list_insert(L, I, q.v[t], q.p[t]);
+    }
+  }
+  q.n = 0;
+}
+
+__device__ __forceinline__ void queue_push(CandQueue& q, bool ins, float s, int32_t p, float (&L)[kList],
+                                           int32_t (&I)[kList]) {
+  if (ins) {
+#pragma unroll
+    for (int t = 0; t < kQueue; ++t) {
+      if (q.n == t) {
+        q.v[t] = s;
+        q.p[t] = p;
+      }
+    }
+    ++q.n;
+  }
+  if (__ballot(q.n == kQueue)) queue_flush(q, L, I);
+}
+
 __global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredParams P) {
   __shared__ __align__(16) __bf16 As[3][kChunk][kDim];
   __shared__ float cen[kDim];
@@ -366,6 +408,8 @@ __global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredPara
     L[r] = -INFINITY;
     I[r] = -1;
   }
+  CandQueue Q;
+  Q.n = 0;
   const double rw = P.radius[wg];
   unsigned long long visited = 0;
   // staging roles: thread t -> point t/4, dims (t%4)*8 .. +7
@@ -373,7 +417,9 @@ __global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredPara
   // the first batch is the workgroup's own chunk alone, so that the thresholds
   // are finite when the other chunks are tested
   for (int64_t v0 = 0, bs = 1; v0 < 2 * P.n_wg; v0 += bs, bs = 64) {
-    // T: the weakest threshold any query of the workgroup still accepts
+    // T: the weakest threshold any query of the workgroup still accepts (the
+    // lists made exact first)
+    queue_flush(Q, L, I);
     float tq = INFINITY;
     {
       float t = nqc - 2.0f * L[kList - 1];
@@ -473,15 +519,13 @@ __global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredPara
         for (int u = 0; u < 4; ++u) {
           const float sv = acc[u];
           const int64_t p = pbase + u;
-          const bool ins = sv > th && p < P.n;
-          if (__ballot(ins)) {
-            if (ins) list_insert(L, I, sv, (int32_t)p);
-          }
+          queue_push(Q, sv > th && p < P.n, sv, (int32_t)p, L, I);
         }
       }
     }
   }
 #undef SG_PREFETCH
+  queue_flush(Q, L, I);
   if (tid == 0 && P.visited) atomicAdd(P.visited, visited);
   if (!qvalid) return;
   int32_t* out = P.cand + qi * kCand + g * kList;
