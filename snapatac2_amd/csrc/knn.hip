@@ -325,7 +325,7 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
 // as for a point that was seen and not kept. Processed chunks are shifted by
 // the centre, given their augmented coordinate |x - c|^2, split into three
 // bf16 terms and staged in LDS; every wave then runs its 16 x 16 MFMA tiles.
-// Accepted candidates of the centred kernel wait in a per-lane queue of kQueue
+// Accepted candidates of the candidate kernels wait in a per-lane queue of kQueue
 // entries and enter the sorted list together. list_insert runs for the whole
 // wave whenever one lane inserts, so inserting from the queues costs one pass
 // per queued entry of the fullest lane instead of one pass per (tile, value) at
@@ -333,7 +333,7 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
 // threshold is the list's older (lower) minimum, which only lets a few more
 // candidates in; list_insert keeps the best kList whatever the order, and a
 // point turned away was below a minimum the final list exceeds.
-constexpr int kQueue = 4;
+constexpr int kQueue = 8;
 
 struct CandQueue {
   float v[kQueue];
@@ -553,6 +553,8 @@ __global__ void __launch_bounds__(256) cand_kernel(CandParams P) {
     I[r] = -1;
   }
   const int64_t ntiles = P.n_pad / 16;
+  CandQueue Q;
+  Q.n = 0;
 
   {
     float qv[kDim];
@@ -573,13 +575,11 @@ __global__ void __launch_bounds__(256) cand_kernel(CandParams P) {
           s = fmaf(x4.z, qv[c4 * 4 + 2], s);
           s = fmaf(x4.w, qv[c4 * 4 + 3], s);
         }
-        const bool ins = s > th && p < P.n;
-        if (__ballot(ins)) {
-          if (ins) list_insert(L, I, s, (int32_t)p);
-        }
+        queue_push(Q, s > th && p < P.n, s, (int32_t)p, L, I);
       }
     }
   }
+  queue_flush(Q, L, I);
   if (!qvalid) return;
   int32_t* out = P.cand + qi * kCand + g * kList;
 #pragma unroll
