@@ -352,8 +352,12 @@ __device__ __forceinline__ void queue_flush(CandQueue& q, float (&L)[kList], int
   q.n = 0;
 }
 
-__device__ __forceinline__ void queue_push(CandQueue& q, bool ins, float s, int32_t p, float (&L)[kList],
-                                           int32_t (&I)[kList]) {
+// Room for the next `room` appends: flush when some lane could overflow.
+__device__ __forceinline__ void queue_reserve(CandQueue& q, int room, float (&L)[kList], int32_t (&I)[kList]) {
+  if (__ballot(q.n > kQueue - room)) queue_flush(q, L, I);
+}
+
+__device__ __forceinline__ void queue_append(CandQueue& q, bool ins, float s, int32_t p) {
   if (ins) {
 #pragma unroll
     for (int t = 0; t < kQueue; ++t) {
@@ -364,7 +368,6 @@ __device__ __forceinline__ void queue_push(CandQueue& q, bool ins, float s, int3
     }
     ++q.n;
   }
-  if (__ballot(q.n == kQueue)) queue_flush(q, L, I);
 }
 
 __global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredParams P) {
@@ -513,13 +516,14 @@ __global__ void __launch_bounds__(kWgWaves * 64) cand_centred_kernel(CentredPara
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bq_h, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_m, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bq_h, acc, 0, 0, 0);
+        queue_reserve(Q, 4, L, I);
         const float th = L[kList - 1];
         const int64_t pbase = c0 + tt * 16 + g * 4;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const float sv = acc[u];
           const int64_t p = pbase + u;
-          queue_push(Q, sv > th && p < P.n, sv, (int32_t)p, L, I);
+          queue_append(Q, sv > th && p < P.n, sv, (int32_t)p);
         }
       }
     }
@@ -561,6 +565,7 @@ __global__ void __launch_bounds__(256) cand_kernel(CandParams P) {
 #pragma unroll
     for (int c = 0; c < kDim; ++c) qv[c] = P.qf[q * kDim + c];
     for (int64_t t = 0; t < ntiles; ++t) {
+      queue_reserve(Q, 4, L, I);
       const float th = L[kList - 1];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -575,7 +580,7 @@ __global__ void __launch_bounds__(256) cand_kernel(CandParams P) {
           s = fmaf(x4.z, qv[c4 * 4 + 2], s);
           s = fmaf(x4.w, qv[c4 * 4 + 3], s);
         }
-        queue_push(Q, s > th && p < P.n, s, (int32_t)p, L, I);
+        queue_append(Q, s > th && p < P.n, s, (int32_t)p);
       }
     }
   }
