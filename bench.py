@@ -305,7 +305,7 @@ def knn_precision(torch, n, k=50, d=30):
 
 
 def embedding_quality(torch, pinfo, emb_learned, g_learned, emb_plain, g_plain, cfg, steps):
-    """Learned against IDF feature weights, judged on what the generator knows:
+    """Learned against IDF feature weights, compared on what the generator knows:
     each cell's type. kNN purity (the fraction of a cell's k neighbours of its
     own type) and type separation (type_separation) of the timed pipeline's
     embedding (weights after one ascent step), of the plain path's (IDF), and,
