@@ -333,7 +333,7 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
 // threshold is the list's older (lower) minimum, which only lets a few more
 // candidates in; list_insert keeps the best kList whatever the order, and a
 // point turned away was below a minimum the final list exceeds.
-constexpr int kQueue = 8;
+constexpr int kQueue = 12;
 
 struct CandQueue {
   float v[kQueue];
