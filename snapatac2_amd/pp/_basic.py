@@ -363,9 +363,12 @@ def _find_most_accessible_features(feature_count, filter_lower_quantile, filter_
     """Same steps as _basic.py:952-967 (numpy's default argsort, so equal counts
     are ordered exactly as the reference orders them on the same host)."""
     idx = np.argsort(feature_count)
-    for i in range(idx.size):
-        if feature_count[idx[i]] > 0:
-            break
+    # first position with a positive count, as the reference's loop finds it (the
+    # last position when none is positive; with no features `i` stays unbound there
+    # and here alike)
+    if idx.size:
+        pos = np.asarray(feature_count)[idx] > 0
+        i = int(np.argmax(pos)) if pos.any() else idx.size - 1
     idx = idx[i:]
     n = idx.size
     n_lower = int(filter_lower_quantile * n)
