@@ -181,11 +181,106 @@ class CapacityError(SnapGpuError):
 
 
 _lib = None
+_handle = None
+
+
+class _Trace:
+    """Per-entry-point GPU time: each library call that returns a status code is
+    bracketed by two HIP events on the calling thread's current stream (the
+    stream every call here launches on), and summary() adds up the time between
+    them once the device is idle."""
+
+    def __init__(self):
+        import threading
+        self._lock = threading.Lock()
+        self._pending = []
+
+    def run(self, name, fn, args):
+        import torch
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rc = fn(*args)
+        e1.record()
+        with self._lock:
+            self._pending.append((name, e0, e1))
+        return rc
+
+    def summary(self) -> dict:
+        """{entry point: {"calls": n, "ms": GPU milliseconds}}, longest first."""
+        out: dict = {}
+        if not self._pending:
+            return out
+        import torch
+        torch.cuda.synchronize()
+        with self._lock:
+            for name, e0, e1 in self._pending:
+                rec = out.setdefault(name, {"calls": 0, "ms": 0.0})
+                rec["calls"] += 1
+                rec["ms"] += e0.elapsed_time(e1)
+        return dict(sorted(out.items(), key=lambda kv: -kv[1]["ms"]))
+
+
+class _TracedLib:
+    """The library with its status-returning entry points timed by a _Trace."""
+
+    def __init__(self, handle, trace):
+        self._handle, self._trace = handle, trace
+
+    def __getattr__(self, name):
+        fn = getattr(self._handle, name)
+        res, args = SIGNATURES.get(name, (None, []))
+        if res is not C.c_int or not args:
+            return fn  # sizes, versions and the error text are not timed
+
+        def call(*args):
+            return self._trace.run(name, fn, args)
+        return call
+
+
+_trace = None
+
+
+def enable_trace() -> None:
+    """Time every library call from now on (what SNAPGPU_TRACE does at load)."""
+    global _trace, _lib
+    lib()
+    if _trace is None:
+        _trace = _Trace()
+        _lib = _TracedLib(_handle, _trace)
+
+
+def disable_trace() -> dict:
+    """Stop timing; returns the summary of the calls timed so far."""
+    global _trace, _lib
+    out = trace_summary()
+    _trace = None
+    _lib = _handle
+    return out
+
+
+def trace_summary() -> dict:
+    return _trace.summary() if _trace is not None else {}
+
+
+def _trace_at_exit(dest: str) -> None:
+    import json
+    import sys
+    rows = trace_summary()
+    if dest in ("1", "stderr"):
+        for name, rec in rows.items():
+            print(f"{rec['ms']:12.3f} ms {rec['calls']:8d}  {name}", file=sys.stderr)
+    else:
+        with open(dest, "w") as f:
+            json.dump(rows, f, indent=1)
 
 
 def lib() -> C.CDLL:
-    global _lib
-    if _lib is None:
+    """The loaded library. With SNAPGPU_TRACE set (1: a table on stderr at exit;
+    a path: JSON written there at exit), every status-returning call is timed
+    on the GPU with HIP events (enable_trace / trace_summary for the same from
+    code)."""
+    global _lib, _handle
+    if _handle is None:
         if not LIB_PATH.exists():
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `python -m snapatac2_amd.build` "
@@ -195,7 +290,13 @@ def lib() -> C.CDLL:
             fn = getattr(handle, name)
             fn.restype = res
             fn.argtypes = args
-        _lib = handle
+        _handle = _lib = handle
+        import os
+        dest = os.environ.get("SNAPGPU_TRACE")
+        if dest:
+            import atexit
+            enable_trace()
+            atexit.register(_trace_at_exit, dest)
     return _lib
 
 

@@ -67,3 +67,19 @@ def test_probe_library_is_separate():
     lib = ctypes.CDLL(str(_lib.LIB_PATH))
     for name in ("snapgpu_copy_bytes", "snapgpu_gather_probe", "snapgpu_probe_gather"):
         assert not hasattr(lib, name)
+
+
+def test_trace_switch_without_calls():
+    """Turning the call trace on and off (no GPU call made): size queries and the
+    version pass through untimed, and the plain library comes back."""
+    plain = _lib.lib()
+    _lib.enable_trace()
+    try:
+        traced = _lib.lib()
+        assert traced is not plain
+        assert traced.snapgpu_abi_version() == plain.snapgpu_abi_version()
+        assert traced.snapgpu_scan_workspace_bytes(1000) == plain.snapgpu_scan_workspace_bytes(1000)
+        assert _lib.trace_summary() == {}
+    finally:
+        assert _lib.disable_trace() == {}
+    assert _lib.lib() is plain
