@@ -511,7 +511,8 @@ __global__ void __launch_bounds__(256) product_f64_kernel(const uint32_t* __rest
                                                           double* __restrict__ partial) {
   constexpr int BW = 2;
   constexpr int G = 64 / BW;
-  constexpr int NB = 4 * BW;  // entries per group batch
+  constexpr int S = 8;         // entry loads per lane per batch
+  constexpr int NB = S * BW;   // entries per group batch (gathers in flight per lane)
   constexpr bool PACK = MODE != 0;
   constexpr uint32_t kFirst = MODE == 1 ? 0x8000u : 0x80000000u;
   const int lane = threadIdx.x & 63;
@@ -533,9 +534,9 @@ __global__ void __launch_bounds__(256) product_f64_kernel(const uint32_t* __rest
   bool have = false;
   double acc = 0.0;
   while (__ballot(pos < end) != 0) {
-    uint32_t e[4], v[4];
+    uint32_t e[S], v[S];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < S; ++s) {
       const int64_t q = pos + s * BW + c;
       e[s] = q < end ? __builtin_nontemporal_load(ent + q) : 0u;
       v[s] = (!PACK && q < end) ? __builtin_nontemporal_load(val + q) : 0u;
