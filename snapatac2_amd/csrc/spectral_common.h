@@ -25,7 +25,7 @@ __device__ __forceinline__ float val32(const uint32_t* v, int64_t e) {
 // together, then its gathers, then the steps, so each lane has kLaneBatch gathers
 // in flight rather than one behind each dependent index load. The order of the
 // steps, and so every rounding, is the plain loop's.
-constexpr int kLaneBatch = 4;
+constexpr int kLaneBatch = 8;
 
 template <bool FV, typename Acc, typename Gather, typename Step>
 __device__ __forceinline__ void lane_entries(const int32_t* __restrict__ idx, const uint32_t* __restrict__ val,
