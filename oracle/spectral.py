@@ -19,9 +19,11 @@ lambda = 1 with eigenvector D^1/2 1, dense numpy.linalg.eigh of the assembled
 operator, the IDF special cases, and bitwise determinism
 (snapatac2-python/tests/test_tools.py:95-122).
 nystrom is pinned by the eigen-equation with every cell a landmark (each output
-row parallel to D^1/2 U + D^-1/2 U L^-1) and by its distance to the exact
-embedding on clustered cells (orthogonalized top-3 subspace within 0.05 rad),
-which an inverted landmark degree scaling fails.
+row parallel to D^1/2 U + D^-1/2 U L^-1; element-wise, the degree step included,
+through sum_j l_j q_j q_j^T = M + I + (M - I)^-1 summed over each row's chunk,
+M = S S^T) and by its distance to the exact embedding on clustered cells
+(orthogonalized top-3 subspace within 0.05 rad), which an inverted landmark
+degree scaling fails.
 """
 from __future__ import annotations
 

@@ -134,6 +134,49 @@ def test_nystrom_oracle_near_exact_embedding():
     assert osp.subspace_angle(oq.real[:, :3], eu[:, :3]) < 0.05
 
 
+@pytest.mark.parametrize("n,m,seed,chunk", [(40, 60, 5, 16), (30, 80, 1, 30), (50, 40, 2, 7)])
+def test_nystrom_oracle_degree_closed_form(n, m, seed, chunk):
+    """nystrom's degree step (embedding.rs:222-229: t = colsum(q) * v over the
+    chunk's rows, d = q t, q /= sqrt(d)) pinned element-wise. With every cell a
+    landmark, M = S S^T (S the normalized rows) and A = D^-1/2 (M - I) D^-1/2,
+    the product of pair j is q_j = D^1/2 u_j + D^-1/2 u_j / l_j (eigen-equation),
+    and the sum of l_j q_j q_j^T over all n pairs is M + I + (M - I)^-1. With
+    the n - 2 pairs of largest |l| that eigsh keeps, d_i is therefore the sum,
+    over the columns of row i's chunk, of M + I + (M - I)^-1 minus the two
+    dropped pairs' terms (taken from numpy's dense eigh). Dropping the
+    eigenvalues from t, or summing q over all rows instead of the chunk, moves d
+    by more than 1e-3 (checked below), against the 1e-10 the oracle meets."""
+    from _clusters import clustered_counts
+    X = clustered_counts(n, m, 3, seed=seed)
+    k = n - 2
+    v, q = osp.nystrom(X, np.arange(n), k, chunk_size=chunk)
+    _, dinv, _, Xn = osp.operator_parts(X)
+    lam, E = np.linalg.eigh(osp.dense_operator(X))
+    order = np.argsort(-np.abs(lam), kind="stable")
+    lam, E = lam[order], E[:, order]
+    sd = np.sqrt(1.0 / dinv)
+    M = (Xn @ Xn.T).toarray()
+    W = M + np.eye(n) + np.linalg.inv(M - np.eye(n))
+    for j in range(k, n):
+        qj = sd * E[:, j] + E[:, j] / sd / lam[j]
+        W -= lam[j] * np.outer(qj, qj)
+    same = (np.arange(n) // chunk)[:, None] == (np.arange(n) // chunk)[None, :]
+    d = (W * same).sum(axis=1)
+    assert d.min() > 0.5  # the d <= 0 clamp stays out of play
+    desc = np.argsort(-lam[:k], kind="stable")  # eigen() returns descending eigenvalues
+    lam_k, E_k = lam[:k][desc], E[:, :k][:, desc]
+    np.testing.assert_allclose(v, lam_k, rtol=0, atol=1e-12)
+    qc = sd[:, None] * E_k + (E_k / sd[:, None]) / lam_k[None, :]
+    sign = np.sign(np.sum(q * qc, axis=0))
+    np.testing.assert_allclose(q, sign * qc / np.sqrt(d)[:, None], rtol=0, atol=1e-10 * np.abs(q).max())
+    # the check separates the plausible mistakes
+    no_eigenvalues = ((qc @ qc.T) * same).sum(axis=1)
+    all_rows = qc @ (qc.sum(axis=0) * lam_k)
+    assert np.abs(no_eigenvalues / d - 1).max() > 1e-3
+    if chunk < n:
+        assert np.abs(all_rows / d - 1).max() > 1e-3
+
+
 def test_nystrom_oracle_pin_rejects_inverted_scaling(monkeypatch):
     X, rows = _nystrom_fixture()
     _, eu = osp.spectral(X, n_comps=6)
