@@ -266,12 +266,16 @@ int snapgpu_spectral_idf(const int64_t* d_colptr, int64_t m, int64_t n, double* 
  * and degrees d = X~ c - 1 (spectral_mf, embedding.rs:133-153); writes
  * dinv = 1/d, r = s*sqrt(dinv) (FP64 and FP32) and w2 = w^2 (FP32).
  * value_type: 0 = uint32 counts, 1 = float32 values (bit patterns in d_cnt).
- * Fails with SNAPGPU_ERR_DATA if a non-empty row has d <= 0. Synchronises. */
+ * Workspace: snapgpu_spectral_prepare_workspace_bytes(m) (the (w_j, c_j) pairs
+ * the degree pass gathers). Fails with SNAPGPU_ERR_DATA if a non-empty row has
+ * d <= 0. Synchronises. */
+int64_t snapgpu_spectral_prepare_workspace_bytes(int64_t m);
 int snapgpu_spectral_prepare(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_cnt,
                              int64_t n, const int64_t* d_colptr, const int32_t* d_rowidx,
                              const uint32_t* d_ccnt, int64_t m, const double* d_w, double* d_s,
                              double* d_c, double* d_dinv, double* d_r64, float* d_r32,
-                             float* d_w2, int32_t value_type, int32_t* d_status, void* stream);
+                             float* d_w2, int32_t value_type, int32_t* d_status, void* d_ws,
+                             int64_t ws_bytes, void* stream);
 
 /* Y = A V for a block of b <= 32 columns (FP64 in/out, FP32 sparse products):
  * A v = r o (X (w^2 o (X^T (r o v)))) - dinv o v, the eigen() matvec

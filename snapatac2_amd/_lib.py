@@ -49,8 +49,9 @@ SIGNATURES: dict[str, tuple] = {
     "snapgpu_csr_transpose": (C.c_int, [_p, _p, _p, _i64, _i64, _i64, _p, _i64, _p, _p, _p, _p,
                                         _i64, _p]),
     "snapgpu_spectral_idf": (C.c_int, [_p, _i64, _i64, _p, _p, _p]),
+    "snapgpu_spectral_prepare_workspace_bytes": (_i64, [_i64]),
     "snapgpu_spectral_prepare": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _p,
-                                           _p, _p, _p, _i32, _p, _p]),
+                                           _p, _p, _p, _i32, _p, _p, _i64, _p]),
     "snapgpu_spectral_apply_workspace_bytes": (_i64, [_i64, _i64]),
     "snapgpu_spectral_apply": (C.c_int, [_p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _p,
                                          _i64, _i32, _p, _i64, _i32, _p, _i64, _p]),

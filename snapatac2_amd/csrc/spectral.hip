@@ -328,21 +328,22 @@ __global__ void __launch_bounds__(256) colsum_kernel(const int64_t* colptr, cons
   if (lane == 0) c[col] = acc * w[col];
 }
 
-// d_i = x~_i . c - 1; dinv = 1/d; r = s sqrt(dinv). Flags rows with d <= 0 that
-// have entries (a cell sharing no feature with any other cell).
+// d_i = x~_i . c - 1; dinv = 1/d; r = s sqrt(dinv). wc[j] = (w_j, c_j): one
+// 16-byte gather per entry. Flags rows with d <= 0 that have entries (a cell
+// sharing no feature with any other cell).
 template <bool FV>
 __global__ void __launch_bounds__(256) degree_kernel(const int64_t* indptr, const int32_t* idx,
-                                                     const uint32_t* cnt, int64_t n, const double* w,
-                                                     const double* s, const double* c, double* dinv,
-                                                     double* r64, float* r32, int32_t* status) {
+                                                     const uint32_t* cnt, int64_t n, const double2* wc,
+                                                     const double* s, double* dinv, double* r64, float* r32,
+                                                     int32_t* status) {
   const int lane = threadIdx.x & 63;
   const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (row >= n) return;
   const int64_t e0 = indptr[row], e1 = indptr[row + 1];
   double acc = 0.0;
   lane_entries<FV>(
-      idx, cnt, e0 + lane, e1, acc, [&](int32_t j) { return make_double2(w[j], c[j]); },
-      [](double& a, double v, double2 wc) { a = fma(v * wc.x, wc.y, a); });
+      idx, cnt, e0 + lane, e1, acc, [&](int32_t j) { return wc[j]; },
+      [](double& a, double v, double2 q) { a = fma(v * q.x, q.y, a); });
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if (lane == 0) {
     const double d = acc * s[row] - 1.0;
@@ -1242,28 +1243,36 @@ int snapgpu_spectral_idf(const int64_t* d_colptr, int64_t m, int64_t n, double* 
   return SNAPGPU_OK;
 }
 
+int64_t snapgpu_spectral_prepare_workspace_bytes(int64_t m) { return m * (int64_t)sizeof(double2) + 256; }
+
 int snapgpu_spectral_prepare(const int64_t* d_indptr, const int32_t* d_idx, const uint32_t* d_cnt,
                              int64_t n, const int64_t* d_colptr, const int32_t* d_rowidx,
                              const uint32_t* d_ccnt, int64_t m, const double* d_w, double* d_s,
                              double* d_c, double* d_dinv, double* d_r64, float* d_r32, float* d_w2,
-                             int32_t value_type, int32_t* d_status, void* stream) {
+                             int32_t value_type, int32_t* d_status, void* d_ws, int64_t ws_bytes,
+                             void* stream) {
   clear_error();
   SG_ARG(n >= 1 && m >= 1, "bad shape");
   SG_ARG(value_type == 0 || value_type == 1, "value_type must be 0 (uint32) or 1 (float32)");
+  SG_ARG(d_ws != nullptr && ws_bytes >= snapgpu_spectral_prepare_workspace_bytes(m), "workspace too small");
   hipStream_t st = as_stream(stream);
+  double2* wc = (double2*)(((uintptr_t)d_ws + 255) & ~(uintptr_t)255);
   SG_HIP(hipMemsetAsync(d_status, 0, 8 * sizeof(int32_t), st));
   const unsigned gn = (unsigned)ceil_div(n, 4), gm = (unsigned)ceil_div(m, 4);
   if (value_type == 0) {
     row_scale_kernel<false><<<gn, 256, 0, st>>>(d_indptr, d_idx, d_cnt, n, d_w, d_s);
     colsum_kernel<false><<<gm, 256, 0, st>>>(d_colptr, d_rowidx, d_ccnt, m, d_w, d_s, d_c);
-    degree_kernel<false><<<gn, 256, 0, st>>>(d_indptr, d_idx, d_cnt, n, d_w, d_s, d_c, d_dinv, d_r64,
-                                              d_r32, d_status);
   } else {
     row_scale_kernel<true><<<gn, 256, 0, st>>>(d_indptr, d_idx, d_cnt, n, d_w, d_s);
     colsum_kernel<true><<<gm, 256, 0, st>>>(d_colptr, d_rowidx, d_ccnt, m, d_w, d_s, d_c);
-    degree_kernel<true><<<gn, 256, 0, st>>>(d_indptr, d_idx, d_cnt, n, d_w, d_s, d_c, d_dinv, d_r64,
-                                             d_r32, d_status);
   }
+  SG_LAUNCH_CHECK();
+  pack_pair_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(d_w, d_c, m, wc);
+  SG_LAUNCH_CHECK();
+  if (value_type == 0)
+    degree_kernel<false><<<gn, 256, 0, st>>>(d_indptr, d_idx, d_cnt, n, wc, d_s, d_dinv, d_r64, d_r32, d_status);
+  else
+    degree_kernel<true><<<gn, 256, 0, st>>>(d_indptr, d_idx, d_cnt, n, wc, d_s, d_dinv, d_r64, d_r32, d_status);
   SG_LAUNCH_CHECK();
   square_f32_kernel<<<(unsigned)ceil_div(m, 256), 256, 0, st>>>(d_w, m, d_w2);
   SG_LAUNCH_CHECK();

@@ -130,6 +130,8 @@ class SpectralOperator:
         self.r32 = torch.empty(n, dtype=torch.float32, device=dev)
         self.w2 = torch.empty(m, dtype=torch.float32, device=dev)
         self._status = torch.zeros(8, dtype=torch.int32, device=dev)
+        self._prep_ws_b = int(L.snapgpu_spectral_prepare_workspace_bytes(m))
+        self._prep_ws = torch.empty(self._prep_ws_b, dtype=torch.uint8, device=dev)
         self._prepare()
         # panel layouts (dense operands larger than one L2-sized panel)
         pc = int(L.snapgpu_panel_count(m))
@@ -304,7 +306,7 @@ class SpectralOperator:
             self.colptr.data_ptr(), self.rowidx.data_ptr(), self.cval.data_ptr(), self.m,
             self.w.data_ptr(), self.s.data_ptr(), self.c.data_ptr(), self.dinv.data_ptr(),
             self.r64.data_ptr(), self.r32.data_ptr(), self.w2.data_ptr(), self.value_type,
-            self._status.data_ptr(), _lib.stream_ptr()))
+            self._status.data_ptr(), self._prep_ws.data_ptr(), self._prep_ws_b, _lib.stream_ptr()))
 
     def set_weights(self, w) -> None:
         """Replace the feature weights and recompute scales and degrees on the device."""
