@@ -58,6 +58,20 @@ def test_gemm_tn_noalloc_rejects_beyond_its_scratch():
     assert L.snapgpu_gemm_tn_f64_noalloc(None, 4, None, 40, 10, 4, 40, None, None) == _lib.ERR_ARG
 
 
+def test_prepare_checks_its_workspace_on_the_host():
+    """The preparation needs m (w_j, c_j) pairs of workspace (16 bytes each, plus
+    alignment); a missing or short workspace is refused before any HIP call."""
+    L = _lib.lib()
+    m = 1000
+    need = int(L.snapgpu_spectral_prepare_workspace_bytes(m))
+    assert need >= 16 * m
+    args = [None, None, None, 10, None, None, None, m] + [None] * 7 + [0, None]
+    assert L.snapgpu_spectral_prepare(*args, None, need, None) == _lib.ERR_ARG
+    assert b"workspace" in L.snapgpu_last_error()
+    assert L.snapgpu_spectral_prepare(*args, 1 << 20, need - 1, None) == _lib.ERR_ARG
+    assert b"workspace" in L.snapgpu_last_error()
+
+
 def test_probe_library_is_separate():
     """The measurement aids live in libsnapgpu_probe.so, not in the product ABI."""
     from snapatac2_amd.build import PROBE_LIB
